@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""FDAS hot-path benchmark on one MI355X.
+
+Metric: DM-trials/sec at N = 2^21 bins, 85 templates, 8 harmonics (4,096
+candidates per harmonic), the single-trial configuration of the reference's
+search.  One step = one DM trial through the whole device path: fused FT
+convolution -> harmonic summing + thresholding + compaction -> per-plane
+selection -> candidate read-back.  Inputs (spectrum, template spectra) are
+resident in HBM before the timed region; two buffer sets A/B keep two trials
+in flight, as the reference's pipelined driver does (main_gtest.cpp:388-437).
+
+Also reported, in the same JSON line:
+  * plain GPU form: one trial per launch, host spectrum uploaded, candidates
+    read back and the stream synchronised per trial (p50 / p99 latency);
+  * roofline of the dominant kernel (HIP-event durations on its own stream);
+  * CPU baseline: the oracle's restatement of the reference path timed on a
+    bounded sample on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ftc_bytes(n_bins, n_templates, tile):
+    """Algorithmic HBM bytes of one FT-convolution launch: spectrum read + FOP write + template spectra."""
+    return 8 * n_bins + 4 * n_templates * n_bins + 8 * n_templates * tile
+
+
+def hsum_bytes(n_bins, n_templates, n_cands):
+    """Algorithmic HBM bytes of one harmonic-summing launch: FOP read once + candidates out."""
+    return 4 * n_templates * n_bins + 8 * n_cands
+
+
+def fft_flops(n_bins, n_templates, tile, overlap):
+    """Nominal 5 S log2 S per transform, forward per tile + inverse per (tile, template)."""
+    payload = tile - overlap
+    n_tiles = -(-n_bins // payload)
+    per = 5.0 * tile * np.log2(tile)
+    return n_tiles * per + n_tiles * n_templates * (per + 6 * tile + 3 * payload)
+
+
+def cpu_baseline(x, bank, thr, cap):
+    """FP64 oracle restatement of ft_conv.py + hsum.py on one full trial (single thread)."""
+    from oracle import fdas_oracle as O
+    t0 = time.perf_counter()
+    fop = O.fop_fft(x, bank)
+    sel, _ = O.detect_streaming(fop, len(thr), thr)
+    O.select(sel, cap)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 4), "unit": "DM-trials/s", "cores": 1, "kind": "port",
+            "sample": f"1 full trial (N=2^21, 85 templates, 8 harmonics) through the FP64 NumPy oracle "
+                      f"(overlap-save FFT convolution + harmonic planes + selection), {dt:.1f} s, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--plain-trials", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile", type=int, default=2048)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the FDAS hot path shards DM trials across GPUs with no "
+              "collective, so a multi-GPU figure is N independent copies of this run (see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import fdas_amd
+    from fdas_amd import synth
+    from fdas_amd.host import FDAS
+
+    w = synth.full_workload()
+    log(f"workload: {w.name}")
+    x = w.spectrum()
+    bank = fdas_amd.make_drift_bank(synth.drift_grid(w.n_templates), w.max_taps)
+    thr = synth.noise_thresholds(x, w.n_harmonics)
+    cap = w.cands_per_harmonic
+
+    f = FDAS(log=lambda lvl, msg: (log(msg) if lvl >= 1 else None))
+    assert f.initialise_accelerator(w.n_bins, tile=args.tile, n_templates=w.n_templates, max_taps=w.max_taps,
+                                    n_harmonics=w.n_harmonics, cands_per_harmonic=cap), f.last_error
+    for ab in (FDAS.A, FDAS.B):
+        assert f.upload_template_taps(bank, ab), f.last_error
+        assert f.perform_input_tiling(x, ab), f.last_error
+    n_list = f.get_candidate_list_sz()
+    loc = {ab: np.zeros(n_list, np.uint32) for ab in (FDAS.A, FDAS.B)}
+    pwr = {ab: np.zeros(n_list, np.float32) for ab in (FDAS.A, FDAS.B)}
+    part = FDAS.AllAccelerations
+
+    def run(n_steps, record):
+        in_flight = []
+        for i in range(n_steps):
+            ab = i % 2
+            if len(in_flight) == 2:
+                prev = in_flight.pop(0)
+                assert f.wait(prev), f.last_error
+                if record is not None:
+                    record.append(f.stage_times(prev))
+            assert f.launch_resident(thr, loc[ab], pwr[ab], part, ab), f.last_error
+            in_flight.append(ab)
+        for prev in in_flight:
+            assert f.wait(prev), f.last_error
+            if record is not None:
+                record.append(f.stage_times(prev))
+
+    log(f"warmup {args.warmup} steps")
+    run(args.warmup, None)
+    stages = []
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    run(args.steps, stages)
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    value = 1000.0 / ms_per_step
+    st = np.array(stages)
+    ftc_ms = float(np.mean(st[:, 1]))
+    hsum_ms = float(np.mean(st[:, 2]))
+    log(f"resident: {ms_per_step:.3f} ms/trial ({value:.1f} trials/s); ftc {ftc_ms:.3f} ms, hsum+select "
+        f"{hsum_ms:.3f} ms")
+
+    # Plain GPU form: one trial per launch, host upload, read-back and synchronisation each time.
+    lat = []
+    for i in range(args.plain_trials + 2):
+        t = time.perf_counter()
+        assert f.launch(x, thr, loc[FDAS.A], pwr[FDAS.A], part, FDAS.A), f.last_error
+        assert f.wait(FDAS.A), f.last_error
+        if i >= 2:
+            lat.append((time.perf_counter() - t) * 1000.0)
+    lat = np.array(lat)
+    plain = {"value": round(float(1000.0 / lat.mean()), 2), "unit": "DM-trials/s",
+             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    log(f"plain: {plain}")
+
+    _, _, _, _, det = f.retrieve_candidates_wide(FDAS.A)
+    n_cands = int(np.minimum(det, cap).sum())
+    geo_overlap = 420
+    b_ftc = ftc_bytes(w.n_bins, w.n_templates, args.tile)
+    flops = fft_flops(w.n_bins, w.n_templates, args.tile, geo_overlap)
+    achieved = b_ftc / (ftc_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "ftconv_kernel<2048>", "kernel_ms": round(ftc_ms, 4),
+                "fft_tflops": round(flops / (ftc_ms * 1e-3) / 1e12, 2), "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+                "hsum_ms": round(hsum_ms, 4),
+                "hsum_gbs": round(hsum_bytes(w.n_bins, w.n_templates, n_cands) / (hsum_ms * 1e-3) / 1e9, 1)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (bounded sample)")
+        cpu = cpu_baseline(x, bank, thr, cap)
+        log(f"cpu: {cpu}")
+
+    f.close()
+    out = {
+        "metric": "DM-trials/sec (2^21 bins, 85 tmpl, 8 harm)",
+        "value": round(value, 2),
+        "unit": "DM-trials/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded N(0,1) series + drifting 8-harmonic tone; 85-template drift bank z=2(j-42))",
+        "config": {"workload": "single-trial FDAS, N=2^21 bins, 85 templates x 421 taps, FFT 2048 overlap-save, "
+                               "8 harmonics, 4096 candidates/harmonic, spectra resident, A/B sets in flight",
+                   "n_bins": w.n_bins, "n_templates": w.n_templates, "tile": args.tile,
+                   "n_harmonics": w.n_harmonics, "cands_per_harmonic": cap},
+        "plain_gpu_form": plain,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,986 @@
+// Host runtime behind the C ABI in include/fdas.h.
+//
+// Mirrors the reference host class FDAS (fdas/src/host/FDAS.cpp): a context
+// owns the device, two buffer sets A/B with their own stream, the resident
+// template spectra, the FOP and the candidate buffers.  Where the reference
+// sequences ~40 OpenCL kernels over on-chip channels, one set here runs three
+// HIP kernels in order on its stream: fused FT convolution, harmonic summing
+// with compaction, and per-plane selection.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fdas_geometry.h"
+#include "fdas_host.h"
+#include "fdas_jobs.h"
+#include "fdas_kernels.h"
+
+using namespace fdas;
+
+namespace {
+
+constexpr uint32_t kDefaultSpill = 262144;
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+enum Ev { EV_START = 0, EV_INPUT, EV_FTC, EV_HSUM, EV_XFER, EV_COUNT };
+
+struct DevSet {
+  float2* input = nullptr;
+  float2* hperm = nullptr;
+  float* fop = nullptr;
+  float2* tiles = nullptr;
+  float* plane = nullptr;
+  uint32_t* spill_key = nullptr;
+  float* spill_pwr = nullptr;
+  uint32_t* counts = nullptr;
+  uint32_t* out_key = nullptr;
+  float* out_pwr = nullptr;
+  uint32_t* out_n = nullptr;
+  FtcJob* d_ftc = nullptr;
+  HsumJob* d_hsum = nullptr;
+  SelectJob* d_sel = nullptr;
+
+  FtcJob* h_ftc = nullptr;
+  HsumJob* h_hsum = nullptr;
+  SelectJob* h_sel = nullptr;
+  uint32_t* h_out_key = nullptr;
+  float* h_out_pwr = nullptr;
+  uint32_t* h_out_n = nullptr;
+  uint32_t* h_counts = nullptr;
+
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[EV_COUNT] = {};
+  hipEvent_t staged_ftc = nullptr;   // h_ftc may be rewritten once this completes
+  hipEvent_t staged_hsum = nullptr;  // h_hsum / h_sel may be rewritten once this completes
+  bool ftc_pending = false;
+  bool hsum_pending = false;
+  bool ev_recorded[EV_COUNT] = {};
+
+  bool templates = false;
+  bool has_input = false;
+  bool fop_valid = false;
+  bool cands_valid = false;
+  bool tiles_valid = false;
+  float thr[kMaxHarmonics] = {};
+  bool thr_set = false;
+
+  bool launched = false;
+  uint32_t* user_loc = nullptr;
+  float* user_pwr = nullptr;
+  bool gated = false;
+  fdas_part gated_part = FDAS_ALL;
+  float gated_thr[kMaxHarmonics] = {};
+};
+
+}  // namespace
+
+struct fdas_ctx {
+  fdas_config cfg{};
+  Geometry geo{};
+  uint32_t c = 0;
+  uint32_t spill = 0;
+  fdas_log_fn log = nullptr;
+  void* user = nullptr;
+  std::string last_error;
+  float2* tw1 = nullptr;
+  float2* tw2 = nullptr;
+  DevSet set[2];
+  hipEvent_t epoch = nullptr;
+  bool sync_pipeline = false;
+  // print_stats state; the reference keeps it file-static (FDAS.cpp:640-641)
+  float pipeline_start = 0.f;
+  float pipeline_accum = 0.f;
+  unsigned pipeline_launches = 0;
+};
+
+namespace {
+
+void emit(fdas_ctx* ctx, int level, const std::string& msg) {
+  if (ctx && ctx->log) ctx->log(level, msg.c_str(), ctx->user);
+}
+
+fdas_status fail(fdas_ctx* ctx, fdas_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) {
+    ctx->last_error = std::string("[ERROR] ") + buf;
+    emit(ctx, FDAS_LOG_ERROR, ctx->last_error);
+  }
+  return st;
+}
+
+void info(fdas_ctx* ctx, const char* fmt, ...) {
+  if (!ctx || !ctx->log) return;
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  emit(ctx, FDAS_LOG_INFO, std::string("[INFO] ") + buf);
+}
+
+#define HIPCK(ctx, expr)                                                                                       \
+  do {                                                                                                         \
+    hipError_t e_ = (expr);                                                                                    \
+    if (e_ != hipSuccess)                                                                                      \
+      return fail(ctx, FDAS_EDEVICE, "HIP call failed with status %d (%s):\n          %s [%s:%d]", int(e_), \
+                  hipGetErrorString(e_), #expr, __FILE__, __LINE__);                                           \
+  } while (0)
+
+#define CHECK_SET(ctx, set)                                                      \
+  do {                                                                           \
+    if (!(ctx)) return FDAS_EINVAL;                                              \
+    if ((set) != FDAS_SET_A && (set) != FDAS_SET_B)                              \
+      return fail(ctx, FDAS_EINVAL, "invalid buffer set %d", int(set));          \
+  } while (0)
+
+bool valid_part(fdas_part p) { return p == FDAS_NEGATIVE || p == FDAS_POSITIVE || p == FDAS_ALL; }
+
+void part_rows(const fdas_ctx* ctx, fdas_part p, uint32_t* begin, uint32_t* count) {
+  switch (p) {
+    case FDAS_NEGATIVE: *begin = 0; *count = ctx->c + 1; break;
+    case FDAS_POSITIVE: *begin = ctx->c; *count = ctx->c + 1; break;
+    default: *begin = 0; *count = ctx->cfg.n_templates; break;
+  }
+}
+
+template <typename T>
+fdas_status dev_alloc(fdas_ctx* ctx, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return FDAS_OK;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(ctx, FDAS_ENOMEM, "device allocation of %zu bytes failed (%s)", count * sizeof(T),
+                hipGetErrorString(e));
+  }
+  return FDAS_OK;
+}
+
+template <typename T>
+fdas_status host_alloc(fdas_ctx* ctx, T** p, size_t count) {
+  *p = nullptr;
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T), 0);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(ctx, FDAS_ENOMEM, "pinned host allocation of %zu bytes failed (%s)", count * sizeof(T),
+                hipGetErrorString(e));
+  }
+  return FDAS_OK;
+}
+
+// Wait until a host job staging buffer may be rewritten (its last copy to the device is done).
+fdas_status wait_staging(fdas_ctx* ctx, hipEvent_t ev, bool& pending) {
+  if (pending) {
+    HIPCK(ctx, hipEventSynchronize(ev));
+    pending = false;
+  }
+  return FDAS_OK;
+}
+
+fdas_status record(fdas_ctx* ctx, DevSet& s, Ev e) {
+  HIPCK(ctx, hipEventRecord(s.ev[e], s.stream));
+  s.ev_recorded[e] = true;
+  return FDAS_OK;
+}
+
+uint32_t ftc_rows_per_item(uint32_t n_rows) {
+  // Enough rows per work item to amortise the forward transform, small enough
+  // for several items per tile (load balance over 256 CUs).
+  const uint32_t target = 16;
+  const uint32_t chunks = std::max<uint32_t>(1, (n_rows + target - 1) / target);
+  return (n_rows + chunks - 1) / chunks;
+}
+
+fdas_status enqueue_ftc(fdas_ctx* ctx, DevSet& s, fdas_part part) {
+  uint32_t rb, nr;
+  part_rows(ctx, part, &rb, &nr);
+  fdas_status st = wait_staging(ctx, s.staged_ftc, s.ftc_pending);
+  if (st != FDAS_OK) return st;
+  FtcJob& j = *s.h_ftc;
+  j = FtcJob{};
+  j.spectrum = s.input;
+  j.fop = s.fop;
+  j.hperm = s.hperm;
+  j.n_bins = ctx->geo.n_bins;
+  j.overlap = ctx->geo.overlap;
+  j.payload = ctx->geo.payload;
+  j.n_tiles = ctx->geo.n_tiles;
+  j.row_begin = rb;
+  j.n_rows = nr;
+  j.rows_per_item = ftc_rows_per_item(nr);
+  j.n_chunks = (nr + j.rows_per_item - 1) / j.rows_per_item;
+  j.item_begin = 0;
+  const uint32_t n_items = j.n_tiles * j.n_chunks;
+  HIPCK(ctx, hipMemcpyAsync(s.d_ftc, s.h_ftc, sizeof(FtcJob), hipMemcpyHostToDevice, s.stream));
+  HIPCK(ctx, hipEventRecord(s.staged_ftc, s.stream));
+  s.ftc_pending = true;
+  HIPCK(ctx, launch_ftconv(int(ctx->cfg.tile), s.d_ftc, 1, n_items, ctx->tw1, ctx->tw2, n_items, s.stream));
+  return FDAS_OK;
+}
+
+fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* thr) {
+  uint32_t rb, nr;
+  part_rows(ctx, part, &rb, &nr);
+  const uint32_t H = ctx->cfg.n_harmonics;
+  fdas_status st = wait_staging(ctx, s.staged_hsum, s.hsum_pending);
+  if (st != FDAS_OK) return st;
+  HsumJob& j = *s.h_hsum;
+  j = HsumJob{};
+  j.fop = s.fop;
+  j.n_bins = ctx->geo.n_bins;
+  j.n_templates = ctx->cfg.n_templates;
+  j.row_begin = rb;
+  j.n_rows = nr;
+  j.n_harmonics = H;
+  j.spill_cap = ctx->spill;
+  j.block_begin = 0;
+  j.spill_key = s.spill_key;
+  j.spill_pwr = s.spill_pwr;
+  j.counts = s.counts;
+  for (uint32_t k = 0; k < H; ++k) j.thr[k] = thr[k];
+  for (uint32_t p = 0; p < H; ++p) {
+    SelectJob& q = s.h_sel[p];
+    q.spill_key = s.spill_key + size_t(p) * ctx->spill;
+    q.spill_pwr = s.spill_pwr + size_t(p) * ctx->spill;
+    q.count = s.counts + p;
+    q.spill_cap = ctx->spill;
+    q.cap = ctx->cfg.cands_per_harmonic;
+    q.out_key = s.out_key + size_t(p) * ctx->cfg.cands_per_harmonic;
+    q.out_pwr = s.out_pwr + size_t(p) * ctx->cfg.cands_per_harmonic;
+    q.out_n = s.out_n + p;
+  }
+  HIPCK(ctx, hipMemcpyAsync(s.d_hsum, s.h_hsum, sizeof(HsumJob), hipMemcpyHostToDevice, s.stream));
+  HIPCK(ctx, hipMemcpyAsync(s.d_sel, s.h_sel, sizeof(SelectJob) * H, hipMemcpyHostToDevice, s.stream));
+  HIPCK(ctx, hipEventRecord(s.staged_hsum, s.stream));
+  s.hsum_pending = true;
+  HIPCK(ctx, launch_zero_u32(s.counts, H, s.stream));
+  const uint32_t blocks = (ctx->geo.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock;
+  HIPCK(ctx, launch_hsum(s.d_hsum, 1, blocks, H, s.stream));
+  HIPCK(ctx, launch_select(s.d_sel, H, s.stream));
+  return FDAS_OK;
+}
+
+fdas_status enqueue_readback(fdas_ctx* ctx, DevSet& s) {
+  const uint32_t H = ctx->cfg.n_harmonics, cap = ctx->cfg.cands_per_harmonic;
+  HIPCK(ctx, hipMemcpyAsync(s.h_out_key, s.out_key, sizeof(uint32_t) * H * cap, hipMemcpyDeviceToHost, s.stream));
+  HIPCK(ctx, hipMemcpyAsync(s.h_out_pwr, s.out_pwr, sizeof(float) * H * cap, hipMemcpyDeviceToHost, s.stream));
+  HIPCK(ctx, hipMemcpyAsync(s.h_out_n, s.out_n, sizeof(uint32_t) * H, hipMemcpyDeviceToHost, s.stream));
+  HIPCK(ctx, hipMemcpyAsync(s.h_counts, s.counts, sizeof(uint32_t) * H, hipMemcpyDeviceToHost, s.stream));
+  return FDAS_OK;
+}
+
+// Converts the staged lists to the reference's 32-bit layout (gen_info.h:62-73).
+void to_reference_lists(const fdas_ctx* ctx, const DevSet& s, uint32_t* loc, float* pwr) {
+  const uint32_t H = ctx->cfg.n_harmonics, cap = ctx->cfg.cands_per_harmonic;
+  const int c = int(ctx->c);
+  const uint32_t inv = invalid_location(c);
+  for (uint32_t p = 0; p < H; ++p) {
+    const uint32_t n = std::min(s.h_out_n[p], cap);
+    for (uint32_t i = 0; i < cap; ++i) {
+      const size_t o = size_t(p) * cap + i;
+      if (i < n) {
+        const uint32_t key = s.h_out_key[o];
+        const int row = int(key >> kKeyBinBits);
+        const uint32_t bin = key & ((1u << kKeyBinBits) - 1);
+        loc[o] = encode_location(p + 1, row - c, bin, c);
+        pwr[o] = s.h_out_pwr[o];
+      } else {
+        loc[o] = inv;
+        pwr[o] = -1.0f;
+      }
+    }
+  }
+}
+
+bool reference_format_ok(const fdas_ctx* ctx) { return ctx->cfg.n_templates <= 127 && ctx->cfg.n_harmonics <= 8; }
+
+fdas_status overflow_status(fdas_ctx* ctx, const DevSet& s) {
+  for (uint32_t p = 0; p < ctx->cfg.n_harmonics; ++p)
+    if (s.h_counts[p] > ctx->spill)
+      return fail(ctx, FDAS_EOVERFLOW,
+                  "plane %u: %u detections exceed the spill capacity %u; lists are incomplete (raise the "
+                  "threshold or spill_per_harmonic)",
+                  p + 1, s.h_counts[p], ctx->spill);
+  return FDAS_OK;
+}
+
+fdas_status flush_gated(fdas_ctx* ctx, int set) {
+  DevSet& s = ctx->set[set];
+  if (!s.gated) return FDAS_OK;
+  s.gated = false;
+  fdas_status st = enqueue_hsum(ctx, s, s.gated_part, s.gated_thr);
+  if (st != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_HSUM)) != FDAS_OK) return st;
+  if ((st = enqueue_readback(ctx, s)) != FDAS_OK) return st;
+  return record(ctx, s, EV_XFER);
+}
+
+void free_set(DevSet& s) {
+  (void)hipFree(s.input);
+  (void)hipFree(s.hperm);
+  (void)hipFree(s.fop);
+  (void)hipFree(s.tiles);
+  (void)hipFree(s.plane);
+  (void)hipFree(s.spill_key);
+  (void)hipFree(s.spill_pwr);
+  (void)hipFree(s.counts);
+  (void)hipFree(s.out_key);
+  (void)hipFree(s.out_pwr);
+  (void)hipFree(s.out_n);
+  (void)hipFree(s.d_ftc);
+  (void)hipFree(s.d_hsum);
+  (void)hipFree(s.d_sel);
+  (void)hipHostFree(s.h_ftc);
+  (void)hipHostFree(s.h_hsum);
+  (void)hipHostFree(s.h_sel);
+  (void)hipHostFree(s.h_out_key);
+  (void)hipHostFree(s.h_out_pwr);
+  (void)hipHostFree(s.h_out_n);
+  (void)hipHostFree(s.h_counts);
+  for (auto& e : s.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s.staged_ftc) (void)hipEventDestroy(s.staged_ftc);
+  if (s.staged_hsum) (void)hipEventDestroy(s.staged_hsum);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+  s = DevSet{};
+}
+
+fdas_status validate_config(fdas_ctx* ctx, const fdas_config& c) {
+  if (c.n_bins == 0 || c.n_bins > kMaxBins)
+    return fail(ctx, FDAS_EINVAL, "n_bins %u outside [1, %u]", c.n_bins, kMaxBins);
+  if (c.n_templates == 0 || (c.n_templates & 1u) == 0 || c.n_templates > kMaxTemplates)
+    return fail(ctx, FDAS_EINVAL, "n_templates %u must be odd and <= %u", c.n_templates, kMaxTemplates);
+  if (c.tile != 1024 && c.tile != 2048 && c.tile != 4096)
+    return fail(ctx, FDAS_EINVAL, "tile %u must be 1024, 2048 or 4096", c.tile);
+  if (c.max_taps == 0 || overlap_for(c.max_taps) >= c.tile)
+    return fail(ctx, FDAS_EINVAL, "max_taps %u leaves no payload in a %u-point tile", c.max_taps, c.tile);
+  if (c.n_harmonics == 0 || c.n_harmonics > kMaxHarmonics)
+    return fail(ctx, FDAS_EINVAL, "n_harmonics %u outside [1, %u]", c.n_harmonics, kMaxHarmonics);
+  if (c.cands_per_harmonic == 0 || c.cands_per_harmonic > kSelectMaxCap)
+    return fail(ctx, FDAS_EINVAL, "cands_per_harmonic %u outside [1, %u]", c.cands_per_harmonic, kSelectMaxCap);
+  return FDAS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" void fdas_default_config(fdas_config* cfg) {
+  if (!cfg) return;
+  *cfg = fdas_config{};
+  cfg->n_bins = 0;
+  cfg->n_templates = 85;
+  cfg->max_taps = 421;
+  cfg->tile = 2048;
+  cfg->n_harmonics = 8;
+  cfg->cands_per_harmonic = 512;
+  cfg->spill_per_harmonic = 0;
+  cfg->device = 0;
+  cfg->flags = 0;
+}
+
+extern "C" const char* fdas_status_string(fdas_status s) {
+  switch (s) {
+    case FDAS_OK: return "ok";
+    case FDAS_EINVAL: return "invalid argument";
+    case FDAS_ENOMEM: return "out of memory";
+    case FDAS_EDEVICE: return "device error";
+    case FDAS_EUNSUPPORTED: return "unsupported";
+    case FDAS_EBUSY: return "busy";
+    case FDAS_ESTATE: return "call out of order";
+    case FDAS_EOVERFLOW: return "candidate spill overflow";
+  }
+  return "unknown status";
+}
+
+extern "C" const char* fdas_last_error(const fdas_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+extern "C" fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void* user, fdas_ctx** out) {
+  if (!cfg || !out) return FDAS_EINVAL;
+  *out = nullptr;
+  fdas_ctx* ctx = new (std::nothrow) fdas_ctx();
+  if (!ctx) return FDAS_ENOMEM;
+  ctx->log = log;
+  ctx->user = user;
+  ctx->cfg = *cfg;
+  fdas_status st = validate_config(ctx, ctx->cfg);
+  if (st != FDAS_OK) {
+    delete ctx;
+    return st;
+  }
+  ctx->geo = make_geometry(cfg->n_bins, cfg->tile, cfg->max_taps);
+  ctx->c = cfg->n_templates / 2;
+  ctx->spill = cfg->spill_per_harmonic ? cfg->spill_per_harmonic : kDefaultSpill;
+  ctx->sync_pipeline = (cfg->flags & FDAS_FLAG_SYNC_PIPELINE) != 0;
+
+  auto bail = [&](fdas_status s) {
+    for (auto& set : ctx->set) free_set(set);
+    (void)hipFree(ctx->tw1);
+    (void)hipFree(ctx->tw2);
+    if (ctx->epoch) (void)hipEventDestroy(ctx->epoch);
+    delete ctx;
+    return s;
+  };
+
+  int n_dev = 0;
+  hipError_t e = hipGetDeviceCount(&n_dev);
+  if (e != hipSuccess || n_dev == 0) {
+    fail(ctx, FDAS_EDEVICE, "no HIP device found (%s)", hipGetErrorString(e));
+    return bail(FDAS_EDEVICE);
+  }
+  if (cfg->device < 0 || cfg->device >= n_dev) {
+    fail(ctx, FDAS_EINVAL, "device %d outside [0, %d)", cfg->device, n_dev);
+    return bail(FDAS_EINVAL);
+  }
+  if ((e = hipSetDevice(cfg->device)) != hipSuccess) {
+    fail(ctx, FDAS_EDEVICE, "hipSetDevice(%d) failed (%s)", cfg->device, hipGetErrorString(e));
+    return bail(FDAS_EDEVICE);
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess)
+    info(ctx, "Device[%d]: %s (%s, %d CUs)", cfg->device, prop.name, prop.gcnArchName, prop.multiProcessorCount);
+
+  // Twiddle tables in FP64, rounded once: w1[k][b] = W_S^(b k), w2[k][d] = W_NT^(d k).
+  const uint32_t S = cfg->tile, NT = S / 16, D = NT / 16;
+  std::vector<float2> tw1(16 * NT), tw2(16 * D);
+  for (uint32_t k = 0; k < 16; ++k) {
+    for (uint32_t b = 0; b < NT; ++b) {
+      const double a = -2.0 * kPi * double((uint64_t(b) * k) % S) / double(S);
+      tw1[k * NT + b] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    }
+    for (uint32_t d = 0; d < D; ++d) {
+      const double a = -2.0 * kPi * double((uint64_t(d) * k) % NT) / double(NT);
+      tw2[k * D + d] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    }
+  }
+  if ((st = dev_alloc(ctx, &ctx->tw1, tw1.size())) != FDAS_OK) return bail(st);
+  if ((st = dev_alloc(ctx, &ctx->tw2, tw2.size())) != FDAS_OK) return bail(st);
+  if (hipMemcpy(ctx->tw1, tw1.data(), tw1.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ctx->tw2, tw2.data(), tw2.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+    fail(ctx, FDAS_EDEVICE, "twiddle upload failed");
+    return bail(FDAS_EDEVICE);
+  }
+  if (hipEventCreate(&ctx->epoch) != hipSuccess) return bail(fail(ctx, FDAS_EDEVICE, "event creation failed"));
+
+  const size_t N = cfg->n_bins, T = cfg->n_templates, H = cfg->n_harmonics, cap = cfg->cands_per_harmonic;
+  for (auto& s : ctx->set) {
+    if ((st = dev_alloc(ctx, &s.input, N)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.hperm, T * S)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.fop, T * N)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.spill_key, H * ctx->spill)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.spill_pwr, H * ctx->spill)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.counts, H)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.out_key, H * cap)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.out_pwr, H * cap)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.out_n, H)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.d_ftc, 1)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.d_hsum, 1)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.d_sel, H)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_ftc, 1)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_hsum, 1)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_sel, H)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_out_key, H * cap)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_out_pwr, H * cap)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_out_n, H)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_counts, H)) != FDAS_OK) return bail(st);
+    std::memset(s.h_out_n, 0, sizeof(uint32_t) * H);
+    std::memset(s.h_counts, 0, sizeof(uint32_t) * H);
+    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(ctx, FDAS_EDEVICE, "stream creation failed"));
+    for (auto& ev : s.ev)
+      if (hipEventCreate(&ev) != hipSuccess) return bail(fail(ctx, FDAS_EDEVICE, "event creation failed"));
+    if (hipEventCreateWithFlags(&s.staged_ftc, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.staged_hsum, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(ctx, FDAS_EDEVICE, "event creation failed"));
+    // a FOP that was never written reads as zeros
+    if (hipMemset(s.fop, 0, T * N * sizeof(float)) != hipSuccess ||
+        hipMemset(s.out_n, 0, H * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(s.counts, 0, H * sizeof(uint32_t)) != hipSuccess)
+      return bail(fail(ctx, FDAS_EDEVICE, "buffer initialisation failed"));
+  }
+  if (hipDeviceSynchronize() != hipSuccess) return bail(fail(ctx, FDAS_EDEVICE, "device synchronisation failed"));
+  (void)hipEventRecord(ctx->epoch, ctx->set[0].stream);
+  info(ctx, "Geometry: N=%u, T=%u, S=%u, overlap=%u, payload=%u, tiles=%u, H=%u, cap=%u, spill=%u", cfg->n_bins,
+       cfg->n_templates, S, ctx->geo.overlap, ctx->geo.payload, ctx->geo.n_tiles, cfg->n_harmonics,
+       cfg->cands_per_harmonic, ctx->spill);
+  *out = ctx;
+  return FDAS_OK;
+}
+
+extern "C" void fdas_destroy(fdas_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->cfg.device);
+  for (auto& s : ctx->set)
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+  for (auto& s : ctx->set) free_set(s);
+  (void)hipFree(ctx->tw1);
+  (void)hipFree(ctx->tw2);
+  if (ctx->epoch) (void)hipEventDestroy(ctx->epoch);
+  delete ctx;
+}
+
+extern "C" uint64_t fdas_get_size(const fdas_ctx* ctx, fdas_size_kind which) {
+  if (!ctx) return 0;
+  switch (which) {
+    case FDAS_SIZE_INPUT: return ctx->geo.n_bins;
+    case FDAS_SIZE_TILES: return uint64_t(ctx->geo.n_tiles) * ctx->cfg.tile;
+    case FDAS_SIZE_TEMPLATES: return uint64_t(ctx->cfg.n_templates) * ctx->cfg.tile;
+    case FDAS_SIZE_THRESHOLDS: return ctx->cfg.n_harmonics;
+    case FDAS_SIZE_FOP: return uint64_t(ctx->cfg.n_templates) * ctx->geo.n_bins;
+    case FDAS_SIZE_CANDIDATES: return uint64_t(ctx->cfg.n_harmonics) * ctx->cfg.cands_per_harmonic;
+  }
+  return 0;
+}
+
+static fdas_status upload_spectra(fdas_ctx* ctx, int set, const std::vector<std::complex<double>>& spec) {
+  // spec: T x S, natural order.  Stored as [t][slot][tid] (kernel register
+  // order after the forward tile transform), scaled by 1/S, which is exact.
+  const uint32_t S = ctx->cfg.tile, NT = S / 16, T = ctx->cfg.n_templates;
+  std::vector<float2> perm(size_t(T) * S);
+  const double scale = 1.0 / double(S);
+  for (uint32_t t = 0; t < T; ++t)
+    for (uint32_t slot = 0; slot < 16; ++slot)
+      for (uint32_t tid = 0; tid < NT; ++tid) {
+        const std::complex<double> v = spec[size_t(t) * S + tile_slot_bin(S, tid, slot)] * scale;
+        perm[size_t(t) * S + size_t(slot) * NT + tid] = make_float2(float(v.real()), float(v.imag()));
+      }
+  DevSet& s = ctx->set[set];
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  HIPCK(ctx, hipStreamSynchronize(s.stream));
+  HIPCK(ctx, hipMemcpy(s.hperm, perm.data(), perm.size() * sizeof(float2), hipMemcpyHostToDevice));
+  s.templates = true;
+  info(ctx, "Uploaded template coefficients");
+  return FDAS_OK;
+}
+
+// Natural bin of entry i of a row stored in 4-parallel FFT order:
+// stored[4m + j] = natural[bitrev2(j) * S/4 + m] (tmpl-gen.py:145-153, ft_conv.py:185-194).
+static inline uint32_t p4_natural_index(uint32_t i, uint32_t S) {
+  const uint32_t j = i & 3u, m = i >> 2;
+  const uint32_t br = ((j & 1u) << 1) | (j >> 1);
+  return br * (S / 4) + m;
+}
+
+extern "C" fdas_status fdas_upload_templates(fdas_ctx* ctx, const float* spectra_c64, int set) {
+  CHECK_SET(ctx, set);
+  if (!spectra_c64) return fail(ctx, FDAS_EINVAL, "null template pointer");
+  const uint32_t S = ctx->cfg.tile, T = ctx->cfg.n_templates;
+  const bool p4 = (ctx->cfg.flags & FDAS_FLAG_REFERENCE_FFT_ORDER) != 0;
+  std::vector<std::complex<double>> spec(size_t(T) * S);
+  for (uint32_t t = 0; t < T; ++t)
+    for (uint32_t i = 0; i < S; ++i) {
+      const float* v = spectra_c64 + 2 * (size_t(t) * S + i);
+      const uint32_t dst = p4 ? p4_natural_index(i, S) : i;
+      spec[size_t(t) * S + dst] = std::complex<double>(v[0], v[1]);
+    }
+  return upload_spectra(ctx, set, spec);
+}
+
+extern "C" fdas_status fdas_upload_template_taps(fdas_ctx* ctx, const float* taps_c64, uint32_t n_taps, int set) {
+  CHECK_SET(ctx, set);
+  if (!taps_c64 || n_taps == 0) return fail(ctx, FDAS_EINVAL, "null or empty template taps");
+  const uint32_t S = ctx->cfg.tile, T = ctx->cfg.n_templates;
+  // Taps beyond max_taps must be zero: the tile overlap covers max_taps - 1 samples only.
+  for (uint32_t t = 0; t < T; ++t)
+    for (uint32_t m = ctx->cfg.max_taps; m < n_taps; ++m) {
+      const float* v = taps_c64 + 2 * (size_t(t) * n_taps + m);
+      if (v[0] != 0.f || v[1] != 0.f)
+        return fail(ctx, FDAS_EINVAL, "template %u has a non-zero tap %u beyond max_taps %u", t, m,
+                    ctx->cfg.max_taps);
+    }
+  std::vector<std::complex<double>> spec(size_t(T) * S);
+  for (uint32_t t = 0; t < T; ++t) {
+    std::complex<double>* row = spec.data() + size_t(t) * S;
+    const uint32_t m_end = std::min(n_taps, ctx->cfg.max_taps);
+    for (uint32_t m = 0; m < m_end; ++m) {
+      const float* v = taps_c64 + 2 * (size_t(t) * n_taps + m);
+      row[m] = std::complex<double>(v[0], v[1]);
+    }
+    fft_inplace(row, S, -1);
+  }
+  return upload_spectra(ctx, set, spec);
+}
+
+extern "C" fdas_status fdas_upload_thresholds(fdas_ctx* ctx, const float* thresholds, int set) {
+  CHECK_SET(ctx, set);
+  if (!thresholds) return fail(ctx, FDAS_EINVAL, "null thresholds");
+  DevSet& s = ctx->set[set];
+  for (uint32_t k = 0; k < ctx->cfg.n_harmonics; ++k) s.thr[k] = thresholds[k];
+  s.thr_set = true;
+  info(ctx, "Uploaded thresholds");
+  return FDAS_OK;
+}
+
+static fdas_status upload_input(fdas_ctx* ctx, DevSet& s, const void* src, hipMemcpyKind kind) {
+  fdas_status st;
+  if ((st = record(ctx, s, EV_START)) != FDAS_OK) return st;
+  HIPCK(ctx, hipMemcpyAsync(s.input, src, sizeof(float2) * ctx->geo.n_bins, kind, s.stream));
+  s.has_input = true;
+  s.fop_valid = false;
+  s.cands_valid = false;
+  s.tiles_valid = false;
+  for (int e = EV_INPUT; e < EV_COUNT; ++e) s.ev_recorded[e] = false;
+  return FDAS_OK;
+}
+
+extern "C" fdas_status fdas_perform_input_tiling(fdas_ctx* ctx, const float* spectrum_c64, int set) {
+  CHECK_SET(ctx, set);
+  if (!spectrum_c64) return fail(ctx, FDAS_EINVAL, "null spectrum");
+  DevSet& s = ctx->set[set];
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  if (!s.tiles && (st = dev_alloc(ctx, &s.tiles, size_t(ctx->geo.n_tiles) * ctx->cfg.tile)) != FDAS_OK) return st;
+  if ((st = upload_input(ctx, s, spectrum_c64, hipMemcpyHostToDevice)) != FDAS_OK) return st;
+  HIPCK(ctx, launch_tiles(int(ctx->cfg.tile), s.input, ctx->geo.n_bins, ctx->geo.overlap, ctx->geo.payload,
+                          ctx->geo.n_tiles, ctx->tw1, ctx->tw2, s.tiles, s.stream));
+  if ((st = record(ctx, s, EV_INPUT)) != FDAS_OK) return st;
+  HIPCK(ctx, hipEventSynchronize(s.ev[EV_INPUT]));
+  s.tiles_valid = true;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, s.ev[EV_START], s.ev[EV_INPUT]);
+  info(ctx, "Preparation duration: %.3f ms", double(ms));
+  return FDAS_OK;
+}
+
+extern "C" fdas_status fdas_set_input_device(fdas_ctx* ctx, const void* d_spectrum_c64, int set) {
+  CHECK_SET(ctx, set);
+  if (!d_spectrum_c64) return fail(ctx, FDAS_EINVAL, "null device spectrum");
+  DevSet& s = ctx->set[set];
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  if ((st = upload_input(ctx, s, d_spectrum_c64, hipMemcpyDeviceToDevice)) != FDAS_OK) return st;
+  return record(ctx, s, EV_INPUT);
+}
+
+static fdas_status check_ready_ftc(fdas_ctx* ctx, const DevSet& s, fdas_part part) {
+  if (!valid_part(part)) return fail(ctx, FDAS_EINVAL, "invalid FOP part %d", int(part));
+  if (!s.templates) return fail(ctx, FDAS_ESTATE, "templates not uploaded for this buffer set");
+  if (!s.has_input) return fail(ctx, FDAS_ESTATE, "no input spectrum in this buffer set");
+  return FDAS_OK;
+}
+
+extern "C" fdas_status fdas_perform_ft_convolution(fdas_ctx* ctx, fdas_part part, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  fdas_status st = check_ready_ftc(ctx, s, part);
+  if (st != FDAS_OK) return st;
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  if ((st = record(ctx, s, EV_INPUT)) != FDAS_OK) return st;
+  if ((st = enqueue_ftc(ctx, s, part)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_FTC)) != FDAS_OK) return st;
+  HIPCK(ctx, hipEventSynchronize(s.ev[EV_FTC]));
+  s.fop_valid = true;
+  s.cands_valid = false;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, s.ev[EV_INPUT], s.ev[EV_FTC]);
+  info(ctx, "FT convolution and IFFT duration: %.3f ms", double(ms));
+  return FDAS_OK;
+}
+
+static const float* pick_thresholds(fdas_ctx* ctx, const DevSet& s, const float* thr) {
+  if (thr) return thr;
+  if (s.thr_set) return s.thr;
+  fail(ctx, FDAS_ESTATE, "no thresholds given or uploaded");
+  return nullptr;
+}
+
+extern "C" fdas_status fdas_perform_harmonic_summing(fdas_ctx* ctx, const float* thresholds, fdas_part part,
+                                                     int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!valid_part(part)) return fail(ctx, FDAS_EINVAL, "invalid FOP part %d", int(part));
+  const float* thr = pick_thresholds(ctx, s, thresholds);
+  if (!thr) return FDAS_ESTATE;
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  if ((st = record(ctx, s, EV_FTC)) != FDAS_OK) return st;
+  if ((st = enqueue_hsum(ctx, s, part, thr)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_HSUM)) != FDAS_OK) return st;
+  if ((st = enqueue_readback(ctx, s)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_XFER)) != FDAS_OK) return st;
+  HIPCK(ctx, hipEventSynchronize(s.ev[EV_XFER]));
+  s.cands_valid = true;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, s.ev[EV_FTC], s.ev[EV_HSUM]);
+  info(ctx, "Harmonic summing duration: %.3f ms", double(ms));
+  return overflow_status(ctx, s);
+}
+
+extern "C" fdas_status fdas_launch(fdas_ctx* ctx, const float* spectrum_c64, const float* thresholds, uint32_t* det_loc,
+                                   float* det_pwr, fdas_part part, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!spectrum_c64 || !det_loc || !det_pwr) return fail(ctx, FDAS_EINVAL, "null launch argument");
+  if (!valid_part(part)) return fail(ctx, FDAS_EINVAL, "invalid FOP part %d", int(part));
+  if (!reference_format_ok(ctx))
+    return fail(ctx, FDAS_EUNSUPPORTED, "T=%u/H=%u do not fit the 32-bit candidate layout; use the wide API",
+                ctx->cfg.n_templates, ctx->cfg.n_harmonics);
+  if (!s.templates) return fail(ctx, FDAS_ESTATE, "templates not uploaded for this buffer set");
+  const float* thr = pick_thresholds(ctx, s, thresholds);
+  if (!thr) return FDAS_ESTATE;
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  // Releases the other set's held harmonic summing (FDAS.cpp:549-550).
+  if (ctx->sync_pipeline && (st = flush_gated(ctx, 1 - set)) != FDAS_OK) return st;
+  if (s.gated && (st = flush_gated(ctx, set)) != FDAS_OK) return st;
+  if ((st = upload_input(ctx, s, spectrum_c64, hipMemcpyHostToDevice)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_INPUT)) != FDAS_OK) return st;
+  if ((st = enqueue_ftc(ctx, s, part)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_FTC)) != FDAS_OK) return st;
+  s.launched = true;
+  s.user_loc = det_loc;
+  s.user_pwr = det_pwr;
+  if (ctx->sync_pipeline) {
+    s.gated = true;
+    s.gated_part = part;
+    for (uint32_t k = 0; k < ctx->cfg.n_harmonics; ++k) s.gated_thr[k] = thr[k];
+    return FDAS_OK;
+  }
+  if ((st = enqueue_hsum(ctx, s, part, thr)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_HSUM)) != FDAS_OK) return st;
+  if ((st = enqueue_readback(ctx, s)) != FDAS_OK) return st;
+  return record(ctx, s, EV_XFER);
+}
+
+extern "C" fdas_status fdas_launch_resident(fdas_ctx* ctx, const float* thresholds, uint32_t* det_loc,
+                                            float* det_pwr, fdas_part part, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!det_loc || !det_pwr) return fail(ctx, FDAS_EINVAL, "null launch argument");
+  if (!valid_part(part)) return fail(ctx, FDAS_EINVAL, "invalid FOP part %d", int(part));
+  if (!reference_format_ok(ctx))
+    return fail(ctx, FDAS_EUNSUPPORTED, "T=%u/H=%u do not fit the 32-bit candidate layout; use the wide API",
+                ctx->cfg.n_templates, ctx->cfg.n_harmonics);
+  if (!s.templates) return fail(ctx, FDAS_ESTATE, "templates not uploaded for this buffer set");
+  if (!s.has_input) return fail(ctx, FDAS_ESTATE, "no input spectrum in this buffer set");
+  const float* thr = pick_thresholds(ctx, s, thresholds);
+  if (!thr) return FDAS_ESTATE;
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  if (ctx->sync_pipeline && (st = flush_gated(ctx, 1 - set)) != FDAS_OK) return st;
+  if (s.gated && (st = flush_gated(ctx, set)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_START)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_INPUT)) != FDAS_OK) return st;
+  if ((st = enqueue_ftc(ctx, s, part)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_FTC)) != FDAS_OK) return st;
+  s.launched = true;
+  s.user_loc = det_loc;
+  s.user_pwr = det_pwr;
+  if (ctx->sync_pipeline) {
+    s.gated = true;
+    s.gated_part = part;
+    for (uint32_t k = 0; k < ctx->cfg.n_harmonics; ++k) s.gated_thr[k] = thr[k];
+    return FDAS_OK;
+  }
+  if ((st = enqueue_hsum(ctx, s, part, thr)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_HSUM)) != FDAS_OK) return st;
+  if ((st = enqueue_readback(ctx, s)) != FDAS_OK) return st;
+  return record(ctx, s, EV_XFER);
+}
+
+extern "C" fdas_status fdas_wait(fdas_ctx* ctx, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!s.launched) return fail(ctx, FDAS_ESTATE, "wait on a buffer set without a launch");
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  if (s.gated && (st = flush_gated(ctx, set)) != FDAS_OK) return st;
+  HIPCK(ctx, hipEventSynchronize(s.ev[EV_XFER]));
+  s.launched = false;
+  s.fop_valid = true;
+  s.cands_valid = true;
+  to_reference_lists(ctx, s, s.user_loc, s.user_pwr);
+  return overflow_status(ctx, s);
+}
+
+extern "C" fdas_status fdas_end_pipeline(fdas_ctx* ctx, int set) {
+  CHECK_SET(ctx, set);
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  return flush_gated(ctx, 1 - set);
+}
+
+extern "C" fdas_status fdas_retrieve_tiles(fdas_ctx* ctx, float* tiles_c64, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!tiles_c64) return fail(ctx, FDAS_EINVAL, "null tiles pointer");
+  if (!s.tiles_valid) return fail(ctx, FDAS_ESTATE, "tiles not materialised (call perform_input_tiling)");
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  const size_t n = size_t(ctx->geo.n_tiles) * ctx->cfg.tile;
+  const bool p4 = (ctx->cfg.flags & FDAS_FLAG_REFERENCE_FFT_ORDER) != 0;
+  if (!p4) {
+    HIPCK(ctx, hipMemcpyAsync(tiles_c64, s.tiles, n * sizeof(float2), hipMemcpyDeviceToHost, s.stream));
+    HIPCK(ctx, hipStreamSynchronize(s.stream));
+    return FDAS_OK;
+  }
+  std::vector<float2> nat(n);
+  HIPCK(ctx, hipMemcpyAsync(nat.data(), s.tiles, n * sizeof(float2), hipMemcpyDeviceToHost, s.stream));
+  HIPCK(ctx, hipStreamSynchronize(s.stream));
+  const uint32_t S = ctx->cfg.tile;
+  for (size_t t = 0; t < ctx->geo.n_tiles; ++t)
+    for (uint32_t i = 0; i < S; ++i) {
+      const float2 v = nat[t * S + p4_natural_index(i, S)];
+      tiles_c64[2 * (t * S + i)] = v.x;
+      tiles_c64[2 * (t * S + i) + 1] = v.y;
+    }
+  return FDAS_OK;
+}
+
+extern "C" fdas_status fdas_retrieve_fop(fdas_ctx* ctx, float* fop, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!fop) return fail(ctx, FDAS_EINVAL, "null FOP pointer");
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  const size_t n = size_t(ctx->cfg.n_templates) * ctx->geo.n_bins;
+  HIPCK(ctx, hipMemcpyAsync(fop, s.fop, n * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+  HIPCK(ctx, hipStreamSynchronize(s.stream));
+  return FDAS_OK;
+}
+
+extern "C" fdas_status fdas_inject_fop(fdas_ctx* ctx, const float* fop, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!fop) return fail(ctx, FDAS_EINVAL, "null FOP pointer");
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  const size_t n = size_t(ctx->cfg.n_templates) * ctx->geo.n_bins;
+  fdas_status st;
+  if ((st = record(ctx, s, EV_START)) != FDAS_OK) return st;
+  if ((st = record(ctx, s, EV_INPUT)) != FDAS_OK) return st;
+  HIPCK(ctx, hipMemcpyAsync(s.fop, fop, n * sizeof(float), hipMemcpyHostToDevice, s.stream));
+  if ((st = record(ctx, s, EV_FTC)) != FDAS_OK) return st;
+  s.fop_valid = true;
+  s.cands_valid = false;
+  return FDAS_OK;
+}
+
+extern "C" fdas_status fdas_retrieve_candidates(fdas_ctx* ctx, uint32_t* det_loc, float* det_pwr, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!det_loc || !det_pwr) return fail(ctx, FDAS_EINVAL, "null candidate pointer");
+  if (!reference_format_ok(ctx))
+    return fail(ctx, FDAS_EUNSUPPORTED, "T=%u/H=%u do not fit the 32-bit candidate layout; use the wide API",
+                ctx->cfg.n_templates, ctx->cfg.n_harmonics);
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  if ((st = enqueue_readback(ctx, s)) != FDAS_OK) return st;
+  HIPCK(ctx, hipStreamSynchronize(s.stream));
+  to_reference_lists(ctx, s, det_loc, det_pwr);
+  return FDAS_OK;
+}
+
+extern "C" fdas_status fdas_retrieve_candidates_wide(fdas_ctx* ctx, fdas_candidate* out, uint32_t capacity,
+                                                     uint32_t* n_written, uint32_t* n_detected, int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!out && capacity) return fail(ctx, FDAS_EINVAL, "null candidate pointer");
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st;
+  if ((st = enqueue_readback(ctx, s)) != FDAS_OK) return st;
+  HIPCK(ctx, hipStreamSynchronize(s.stream));
+  const uint32_t H = ctx->cfg.n_harmonics, cap = ctx->cfg.cands_per_harmonic;
+  const int c = int(ctx->c);
+  uint32_t w = 0;
+  for (uint32_t p = 0; p < H; ++p) {
+    if (n_detected) n_detected[p] = s.h_counts[p];
+    const uint32_t n = std::min(s.h_out_n[p], cap);
+    for (uint32_t i = 0; i < n && w < capacity; ++i, ++w) {
+      const uint32_t key = s.h_out_key[size_t(p) * cap + i];
+      out[w].harmonic = p + 1;
+      out[w].template_num = int(key >> kKeyBinBits) - c;
+      out[w].bin = key & ((1u << kKeyBinBits) - 1);
+      out[w].power = s.h_out_pwr[size_t(p) * cap + i];
+    }
+  }
+  if (n_written) *n_written = w;
+  return overflow_status(ctx, s);
+}
+
+extern "C" fdas_status fdas_retrieve_harmonic_plane(fdas_ctx* ctx, uint32_t k, float* plane, fdas_part part,
+                                                    int set) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!plane) return fail(ctx, FDAS_EINVAL, "null plane pointer");
+  if (k == 0 || k > ctx->cfg.n_harmonics) return fail(ctx, FDAS_EINVAL, "harmonic %u outside [1, %u]", k,
+                                                      ctx->cfg.n_harmonics);
+  if (!valid_part(part)) return fail(ctx, FDAS_EINVAL, "invalid FOP part %d", int(part));
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  const size_t n = size_t(ctx->cfg.n_templates) * ctx->geo.n_bins;
+  fdas_status st;
+  if (!s.plane && (st = dev_alloc(ctx, &s.plane, n)) != FDAS_OK) return st;
+  uint32_t rb, nr;
+  part_rows(ctx, part, &rb, &nr);
+  HIPCK(ctx, launch_hplane(s.fop, ctx->geo.n_bins, ctx->cfg.n_templates, rb, nr, k, s.plane, s.stream));
+  HIPCK(ctx, hipMemcpyAsync(plane + size_t(rb) * ctx->geo.n_bins, s.plane + size_t(rb) * ctx->geo.n_bins,
+                            size_t(nr) * ctx->geo.n_bins * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+  HIPCK(ctx, hipStreamSynchronize(s.stream));
+  return FDAS_OK;
+}
+
+static float elapsed(const DevSet& s, int a, int b) {
+  if (!s.ev_recorded[a] || !s.ev_recorded[b]) return 0.f;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, s.ev[a], s.ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.f;
+  }
+  return ms;
+}
+
+extern "C" fdas_status fdas_get_stage_times(fdas_ctx* ctx, int set, float* ms5) {
+  CHECK_SET(ctx, set);
+  if (!ms5) return fail(ctx, FDAS_EINVAL, "null output");
+  const DevSet& s = ctx->set[set];
+  ms5[0] = elapsed(s, EV_START, EV_INPUT);
+  ms5[1] = elapsed(s, EV_INPUT, EV_FTC);
+  ms5[2] = elapsed(s, EV_FTC, EV_HSUM);
+  ms5[3] = elapsed(s, EV_HSUM, EV_XFER);
+  ms5[4] = elapsed(s, EV_START, EV_XFER);
+  return FDAS_OK;
+}
+
+extern "C" void fdas_print_stats(fdas_ctx* ctx, int set, int reset) {
+  if (!ctx || (set != FDAS_SET_A && set != FDAS_SET_B)) return;
+  const DevSet& s = ctx->set[set];
+  float t0 = 0.f, t1 = 0.f;
+  (void)hipEventElapsedTime(&t0, ctx->epoch, s.ev[EV_START]);
+  (void)hipEventElapsedTime(&t1, ctx->epoch, s.ev[EV_XFER]);
+  if (reset) {
+    ctx->pipeline_start = t0;
+    ctx->pipeline_accum = 0.f;
+    ctx->pipeline_launches = 1;
+  } else {
+    ++ctx->pipeline_launches;
+  }
+  const float acc = t1 - ctx->pipeline_start;
+  info(ctx, "Launch  #%u%c complete @ %.3f ms II ~ %.3f ms", ctx->pipeline_launches, set == FDAS_SET_A ? 'A' : 'B',
+       double(acc), double(acc - ctx->pipeline_accum));
+  info(ctx, "  Preparation duration: %.3f ms", double(elapsed(s, EV_START, EV_INPUT)));
+  info(ctx, "  FT convolution and IFFT duration: %.3f ms", double(elapsed(s, EV_INPUT, EV_FTC)));
+  info(ctx, "  Harmonic summing duration: %.3f ms", double(elapsed(s, EV_FTC, EV_HSUM)));
+  info(ctx, "  Total duration: %.3f ms", double(elapsed(s, EV_START, EV_XFER)));
+  ctx->pipeline_accum = acc;
+}
+
+extern "C" void fdas_print_events(fdas_ctx* ctx, int set) {
+  if (!ctx || !ctx->log || (set != FDAS_SET_A && set != FDAS_SET_B)) return;
+  const DevSet& s = ctx->set[set];
+  float t[EV_COUNT] = {};
+  for (int e = 0; e < EV_COUNT; ++e)
+    if (s.ev_recorded[e]) (void)hipEventElapsedTime(&t[e], ctx->epoch, s.ev[e]);
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "@@@,%c,%.6f,%.6f,%.6f,%.6f,%.6f", set == FDAS_SET_A ? 'A' : 'B', double(t[0]),
+                double(t[1]), double(t[2]), double(t[3]), double(t[4]));
+  emit(ctx, FDAS_LOG_INFO, buf);
+}

@@ -1,0 +1,41 @@
+// Host-side entry points of the device kernels (implemented in fdas_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fdas_jobs.h"
+
+namespace fdas {
+
+constexpr int kHsumThreads = 256;
+constexpr int kHsumBinsPerBlock = 2048;   // bins one HSUM workgroup covers (all rows of its part)
+constexpr int kHsumStage = 256;           // LDS staging entries per plane before a global flush
+constexpr int kSelectThreads = 1024;
+constexpr uint32_t kSelectMaxCap = 8192;  // largest per-plane cap the in-LDS sort handles
+
+int ftconv_threads(int tile);
+
+// Fused FT convolution over a list of jobs (all with the same tile size).
+hipError_t launch_ftconv(int tile, const FtcJob* d_jobs, uint32_t n_jobs, uint32_t n_items, const float2* tw1,
+                         const float2* tw2, uint32_t grid, hipStream_t stream);
+
+// Forward transform of every tile of one spectrum, natural bin order (debug / drop-in retrieve_tiles).
+hipError_t launch_tiles(int tile, const float2* spectrum, uint32_t n_bins, uint32_t overlap, uint32_t payload,
+                        uint32_t n_tiles, const float2* tw1, const float2* tw2, float2* out, hipStream_t stream);
+
+// Harmonic summing + thresholding + compaction into per-plane spill lists.
+hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, uint32_t max_harmonics,
+                       hipStream_t stream);
+
+// Per-plane cap (strongest first) and sort by (row, bin); one workgroup per job.
+hipError_t launch_select(const SelectJob* d_jobs, uint32_t n_jobs, hipStream_t stream);
+
+// Harmonic plane HP_k over rows [row_begin, row_begin + n_rows) (debug / parity).
+hipError_t launch_hplane(const float* fop, uint32_t n_bins, uint32_t n_templates, uint32_t row_begin,
+                         uint32_t n_rows, uint32_t k, float* out, hipStream_t stream);
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
+
+}  // namespace fdas

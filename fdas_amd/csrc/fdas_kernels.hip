@@ -1,0 +1,480 @@
+// Device kernels of the FDAS hot path for CDNA4 (gfx950).
+//
+//  ftconv_kernel  fused overlap-save FT convolution: tile load -> forward FFT
+//                 -> x template spectrum -> inverse FFT -> discard overlap -> |x|^2
+//                 (reference: tile_input.cl.mako, fft.cl.mako, mux_and_mult.cl.mako,
+//                  square_and_discard.cl.mako; oracle ft_conv.py:158-207)
+//  hsum_kernel    harmonic summing + per-plane strict thresholding + compaction
+//                 (reference: hms_baseline.cl.mako:55-71, detect.cl.mako:81-104,
+//                  hsum.py:91-151)
+//  select_kernel  per-plane cap (strongest first) and (row, bin) ordering of the
+//                 detections; replaces the reference's overwrite-oldest ring
+//                 (detect.cl.mako:106-115) with a deterministic rule.
+#include <hip/hip_runtime.h>
+
+#include "fdas_kernels.h"
+#include "fft_tile.h"
+
+namespace fdas {
+
+// ---------------------------------------------------------------------------
+// FT convolution
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t find_job_ftc(const FtcJob* __restrict__ jobs, uint32_t n_jobs, uint32_t item) {
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (jobs[mid].item_begin <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int S>
+__global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict__ jobs, uint32_t n_jobs,
+                                                        uint32_t n_items, const float2* __restrict__ tw1,
+                                                        const float2* __restrict__ tw2) {
+  using F = TileFFT<S>;
+  constexpr int NT = F::NT;
+  __shared__ float2 la[F::LDS_ELEMS];
+  __shared__ float2 lb[F::LDS_ELEMS];
+  const int tid = threadIdx.x;
+  F fft;
+  fft.load_twiddles(tw1, tw2, tid);
+
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const FtcJob* job = jobs + find_job_ftc(jobs, n_jobs, item);
+    const uint32_t n_bins = job->n_bins;
+    const uint32_t overlap = job->overlap;
+    const uint32_t payload = job->payload;
+    const uint32_t local = item - job->item_begin;
+    const uint32_t chunk = local / job->n_tiles;  // consecutive items share a template chunk
+    const uint32_t tile = local - chunk * job->n_tiles;
+    const float2* __restrict__ x = static_cast<const float2*>(job->spectrum);
+
+    __syncthreads();  // previous item's LDS readers are done
+
+    float2 xf[16];
+    const int base = int(tile * payload) - int(overlap);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int idx = base + NT * a + tid;
+      xf[a] = (idx >= 0 && idx < int(n_bins)) ? x[idx] : make_float2(0.f, 0.f);
+    }
+    fft.forward(xf, la, tid);
+
+    const uint32_t r_begin = job->row_begin + chunk * job->rows_per_item;
+    uint32_t r_end = r_begin + job->rows_per_item;
+    const uint32_t r_stop = job->row_begin + job->n_rows;
+    if (r_end > r_stop) r_end = r_stop;
+    const float2* __restrict__ hbase = static_cast<const float2*>(job->hperm);
+    const uint32_t out0 = tile * payload;  // FOP bin of tile sample s = overlap
+
+    for (uint32_t r = r_begin; r < r_end; ++r) {
+      const float2* __restrict__ h = hbase + size_t(r) * S + tid;
+      float2 v[16];
+#pragma unroll
+      for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], h[s * NT]);
+      fft.inverse(v, la, lb, tid);
+      float* __restrict__ out = job->fop + size_t(r) * n_bins;
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        const uint32_t s = uint32_t(NT * a + tid);
+        if (s >= overlap) {
+          const uint32_t f = out0 + s - overlap;
+          if (f < n_bins) out[f] = v[a].x * v[a].x + v[a].y * v[a].y;
+        }
+      }
+    }
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(S / 16) void tiles_kernel(const float2* __restrict__ x, uint32_t n_bins,
+                                                       uint32_t overlap, uint32_t payload,
+                                                       const float2* __restrict__ tw1,
+                                                       const float2* __restrict__ tw2, float2* __restrict__ out) {
+  using F = TileFFT<S>;
+  constexpr int NT = F::NT;
+  __shared__ float2 la[F::LDS_ELEMS];
+  const int tid = threadIdx.x;
+  F fft;
+  fft.load_twiddles(tw1, tw2, tid);
+  const uint32_t tile = blockIdx.x;
+  const int base = int(tile * payload) - int(overlap);
+  float2 v[16];
+#pragma unroll
+  for (int a = 0; a < 16; ++a) {
+    const int idx = base + NT * a + tid;
+    v[a] = (idx >= 0 && idx < int(n_bins)) ? x[idx] : make_float2(0.f, 0.f);
+  }
+  fft.forward(v, la, tid);
+  float2* __restrict__ o = out + size_t(tile) * S;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) o[F::slot_bin(tid, s)] = v[s];
+}
+
+int ftconv_threads(int tile) { return tile / 16; }
+
+hipError_t launch_ftconv(int tile, const FtcJob* d_jobs, uint32_t n_jobs, uint32_t n_items, const float2* tw1,
+                         const float2* tw2, uint32_t grid, hipStream_t stream) {
+  if (n_items == 0) return hipSuccess;
+  if (grid == 0 || grid > n_items) grid = n_items;
+  switch (tile) {
+    case 1024: hipLaunchKernelGGL(ftconv_kernel<1024>, dim3(grid), dim3(64), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
+    case 2048: hipLaunchKernelGGL(ftconv_kernel<2048>, dim3(grid), dim3(128), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
+    case 4096: hipLaunchKernelGGL(ftconv_kernel<4096>, dim3(grid), dim3(256), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_tiles(int tile, const float2* spectrum, uint32_t n_bins, uint32_t overlap, uint32_t payload,
+                        uint32_t n_tiles, const float2* tw1, const float2* tw2, float2* out, hipStream_t stream) {
+  if (n_tiles == 0) return hipSuccess;
+  switch (tile) {
+    case 1024: hipLaunchKernelGGL(tiles_kernel<1024>, dim3(n_tiles), dim3(64), 0, stream, spectrum, n_bins, overlap, payload, tw1, tw2, out); break;
+    case 2048: hipLaunchKernelGGL(tiles_kernel<2048>, dim3(n_tiles), dim3(128), 0, stream, spectrum, n_bins, overlap, payload, tw1, tw2, out); break;
+    case 4096: hipLaunchKernelGGL(tiles_kernel<4096>, dim3(n_tiles), dim3(256), 0, stream, spectrum, n_bins, overlap, payload, tw1, tw2, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Harmonic summing
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+struct HsumStage {
+  uint32_t key[kMaxHarmonics][kHsumStage];
+  float pwr[kMaxHarmonics][kHsumStage];
+  uint32_t n[kMaxHarmonics];
+  uint32_t base[kMaxHarmonics];
+};
+
+// Appends the lanes with `hit` set to plane p's LDS stage; entries beyond the
+// stage go straight to the global spill list.
+__device__ __forceinline__ void stage_append(HsumStage& st, const HsumJob& job, uint32_t p, bool hit, uint32_t key,
+                                             float pwr) {
+  const uint64_t mask = __ballot(hit);
+  if (mask == 0) return;
+  const uint32_t lane = lane_id();
+  const uint32_t leader = uint32_t(__builtin_ctzll(mask));
+  uint32_t pos = 0;
+  if (lane == leader) pos = atomicAdd(&st.n[p], uint32_t(__builtin_popcountll(mask)));
+  pos = __shfl(pos, int(leader)) + lanes_below(mask);
+  if (!hit) return;
+  if (pos < uint32_t(kHsumStage)) {
+    st.key[p][pos] = key;
+    st.pwr[p][pos] = pwr;
+  } else {
+    const uint32_t g = atomicAdd(&job.counts[p], 1u);
+    if (g < job.spill_cap) {
+      job.spill_key[size_t(p) * job.spill_cap + g] = key;
+      job.spill_pwr[size_t(p) * job.spill_cap + g] = pwr;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t find_job_hsum(const HsumJob* __restrict__ jobs, uint32_t n_jobs, uint32_t blk) {
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (jobs[mid].block_begin <= blk) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int HMAX>
+__global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __restrict__ jobs, uint32_t n_jobs) {
+  __shared__ HsumStage st;
+  const HsumJob& job = jobs[find_job_hsum(jobs, n_jobs, blockIdx.x)];
+  const int tid = threadIdx.x;
+  const uint32_t H = job.n_harmonics;
+  if (tid < int(kMaxHarmonics)) st.n[tid] = 0;
+  __syncthreads();
+
+  const uint32_t n_bins = job.n_bins;
+  const int c = int(job.n_templates / 2);
+  const float* __restrict__ fop = job.fop;
+  const uint32_t f_block = (blockIdx.x - job.block_begin) * uint32_t(kHsumBinsPerBlock);
+
+  for (int sub = 0; sub < kHsumBinsPerBlock / kHsumThreads; ++sub) {
+    const uint32_t f = f_block + uint32_t(sub * kHsumThreads + tid);
+    if (f_block + uint32_t(sub * kHsumThreads) >= n_bins) break;  // uniform
+    const bool valid = f < n_bins;
+    const uint32_t fs = valid ? f : 0;
+    for (uint32_t r = job.row_begin; r < job.row_begin + job.n_rows; ++r) {
+      const int t = int(r) - c;
+      float hp = valid ? fop[size_t(r) * n_bins + fs] : 0.f;
+      const uint32_t key = (r << kKeyBinBits) | fs;
+      stage_append(st, job, 0, valid && hp > job.thr[0], key, hp);
+#pragma unroll
+      for (int k = 2; k <= HMAX; ++k) {
+        if (uint32_t(k) <= H) {
+          const int rk = c + t / k;
+          hp += valid ? fop[size_t(rk) * n_bins + fs / uint32_t(k)] : 0.f;
+          stage_append(st, job, uint32_t(k - 1), valid && hp > job.thr[k - 1], key, hp);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < int(H)) {
+    const uint32_t n = min(st.n[tid], uint32_t(kHsumStage));
+    st.base[tid] = n ? atomicAdd(&job.counts[tid], n) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t p = 0; p < H; ++p) {
+    const uint32_t n = min(st.n[p], uint32_t(kHsumStage));
+    const uint32_t base = st.base[p];
+    for (uint32_t i = tid; i < n; i += kHsumThreads) {
+      const uint32_t g = base + i;
+      if (g < job.spill_cap) {
+        job.spill_key[size_t(p) * job.spill_cap + g] = st.key[p][i];
+        job.spill_pwr[size_t(p) * job.spill_cap + g] = st.pwr[p][i];
+      }
+    }
+  }
+}
+
+hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, uint32_t max_harmonics,
+                       hipStream_t stream) {
+  if (n_blocks == 0) return hipSuccess;
+  if (max_harmonics <= 8)
+    hipLaunchKernelGGL(hsum_kernel<8>, dim3(n_blocks), dim3(kHsumThreads), 0, stream, d_jobs, n_jobs);
+  else if (max_harmonics <= 16)
+    hipLaunchKernelGGL(hsum_kernel<16>, dim3(n_blocks), dim3(kHsumThreads), 0, stream, d_jobs, n_jobs);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void hplane_kernel(const float* __restrict__ fop, uint32_t n_bins,
+                                                     uint32_t n_templates, uint32_t row_begin, uint32_t k_max,
+                                                     float* __restrict__ out) {
+  const uint32_t f = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t r = row_begin + blockIdx.y;
+  if (f >= n_bins) return;
+  const int c = int(n_templates / 2);
+  const int t = int(r) - c;
+  float hp = fop[size_t(r) * n_bins + f];
+  for (uint32_t k = 2; k <= k_max; ++k) {
+    const int rk = c + t / int(k);
+    hp += fop[size_t(rk) * n_bins + f / k];
+  }
+  out[size_t(r) * n_bins + f] = hp;
+}
+
+hipError_t launch_hplane(const float* fop, uint32_t n_bins, uint32_t n_templates, uint32_t row_begin, uint32_t n_rows,
+                         uint32_t k, float* out, hipStream_t stream) {
+  if (n_rows == 0 || n_bins == 0) return hipSuccess;
+  dim3 grid((n_bins + 255) / 256, n_rows);
+  hipLaunchKernelGGL(hplane_kernel, grid, dim3(256), 0, stream, fop, n_bins, n_templates, row_begin, k, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Selection: keep the `cap` strongest detections of a plane (ties: smaller
+// (row, bin) key first), returned sorted by key.
+// ---------------------------------------------------------------------------
+
+// Exclusive prefix sum over the workgroup (kSelectThreads threads); returns the total in *total.
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_waves, uint32_t* total) {
+  const int tid = threadIdx.x;
+  const int lane = int(lane_id());
+  const int wave = tid >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_waves[wave] = x;
+  __syncthreads();
+  if (tid < 64) {
+    const int n_waves = kSelectThreads / 64;
+    uint32_t w = tid < n_waves ? s_waves[tid] : 0u;
+    uint32_t ws = w;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(ws, o);
+      if (lane >= o) ws += y;
+    }
+    if (tid < n_waves) s_waves[tid] = ws - w;
+    if (tid == n_waves - 1) s_waves[32] = ws;
+  }
+  __syncthreads();
+  const uint32_t res = s_waves[wave] + x - v;
+  *total = s_waves[32];
+  __syncthreads();
+  return res;
+}
+
+struct SelectShared {
+  uint32_t key[kSelectMaxCap];
+  float pwr[kSelectMaxCap];
+  uint32_t hist[2048];
+  uint32_t waves[40];
+  uint32_t misc[8];
+};
+
+// One radix-select digit pass: among values v with (v & mask) == prefix, find
+// the digit (bits [shift, shift+width)) holding the `rank`-th element counted
+// from the top (descending) or bottom (ascending).  Updates prefix/mask/rank.
+template <bool DESCENDING, typename Get>
+__device__ void radix_pass(SelectShared& sh, uint32_t n, Get get, uint32_t shift, uint32_t width, uint32_t& prefix,
+                           uint32_t& mask, uint32_t& rank) {
+  const int tid = threadIdx.x;
+  const uint32_t nb = 1u << width;
+  for (uint32_t i = tid; i < 2048; i += kSelectThreads) sh.hist[i] = 0;
+  __syncthreads();
+  for (uint32_t e = tid; e < n; e += kSelectThreads) {
+    const uint32_t v = get(e);
+    if ((v & mask) == prefix) atomicAdd(&sh.hist[(v >> shift) & (nb - 1)], 1u);
+  }
+  __syncthreads();
+  // Thread t owns ordered positions u = 2t, 2t+1 (u counts from the selected end).
+  uint32_t h[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t u = uint32_t(2 * tid + j);
+    const uint32_t d = DESCENDING ? (nb - 1 - u) : u;
+    h[j] = u < nb ? sh.hist[d] : 0u;
+  }
+  uint32_t total;
+  const uint32_t excl = block_exclusive_scan(h[0] + h[1], sh.waves, &total);
+  if (excl < rank && rank <= excl + h[0] + h[1]) {
+    uint32_t u = uint32_t(2 * tid);
+    uint32_t r = rank - excl;
+    if (r > h[0]) {
+      r -= h[0];
+      u += 1;
+    }
+    const uint32_t d = DESCENDING ? (nb - 1 - u) : u;
+    sh.misc[0] = d;
+    sh.misc[1] = r;
+    sh.misc[2] = sh.hist[d];
+  }
+  __syncthreads();
+  prefix |= sh.misc[0] << shift;
+  mask |= (nb - 1) << shift;
+  rank = sh.misc[1];
+  __syncthreads();
+}
+
+__device__ void bitonic_sort_by_key(SelectShared& sh, uint32_t n) {
+  const int tid = threadIdx.x;
+  uint32_t p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  for (uint32_t i = n + tid; i < p2; i += kSelectThreads) {
+    sh.key[i] = 0xffffffffu;
+    sh.pwr[i] = 0.f;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < p2; i += kSelectThreads) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint32_t a = sh.key[i], b = sh.key[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            sh.key[i] = b;
+            sh.key[ixj] = a;
+            const float pa = sh.pwr[i];
+            sh.pwr[i] = sh.pwr[ixj];
+            sh.pwr[ixj] = pa;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(kSelectThreads) void select_kernel(const SelectJob* __restrict__ jobs) {
+  __shared__ SelectShared sh;
+  const SelectJob& job = jobs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const uint32_t n_det = *job.count;
+  const uint32_t n = min(n_det, job.spill_cap);
+  const uint32_t cap = job.cap;
+  const uint32_t* __restrict__ skey = job.spill_key;
+  const float* __restrict__ spwr = job.spill_pwr;
+
+  uint32_t kept;
+  if (n <= cap) {
+    for (uint32_t e = tid; e < n; e += kSelectThreads) {
+      sh.key[e] = skey[e];
+      sh.pwr[e] = spwr[e];
+    }
+    kept = n;
+  } else {
+    // Radix select on the power bits (non-negative floats order like their bits).
+    auto pbits = [&](uint32_t e) { return __float_as_uint(spwr[e]); };
+    uint32_t prefix = 0, mask = 0, rank = cap;
+    radix_pass<true>(sh, n, pbits, 21, 11, prefix, mask, rank);
+    radix_pass<true>(sh, n, pbits, 10, 11, prefix, mask, rank);
+    radix_pass<true>(sh, n, pbits, 0, 10, prefix, mask, rank);
+    const uint32_t vbits = prefix;
+    const uint32_t n_equal = sh.misc[2];
+    // `rank` entries with power == V are kept: the smallest keys among them.
+    uint32_t kmax = 0xffffffffu;
+    if (n_equal > rank) {
+      auto kbits = [&](uint32_t e) { return __float_as_uint(spwr[e]) == vbits ? skey[e] : 0xffffffffu; };
+      uint32_t kp = 0, km = 0, kr = rank;
+      radix_pass<false>(sh, n, kbits, 21, 11, kp, km, kr);
+      radix_pass<false>(sh, n, kbits, 10, 11, kp, km, kr);
+      radix_pass<false>(sh, n, kbits, 0, 10, kp, km, kr);
+      kmax = kp;
+    }
+    if (tid == 0) sh.misc[3] = 0;
+    __syncthreads();
+    for (uint32_t e = tid; e < n; e += kSelectThreads) {
+      const uint32_t pb = __float_as_uint(spwr[e]);
+      const uint32_t key = skey[e];
+      if (pb > vbits || (pb == vbits && key <= kmax)) {
+        const uint32_t pos = atomicAdd(&sh.misc[3], 1u);
+        if (pos < cap) {
+          sh.key[pos] = key;
+          sh.pwr[pos] = spwr[e];
+        }
+      }
+    }
+    __syncthreads();
+    kept = min(sh.misc[3], cap);
+  }
+  __syncthreads();
+  bitonic_sort_by_key(sh, kept);
+  for (uint32_t e = tid; e < kept; e += kSelectThreads) {
+    job.out_key[e] = sh.key[e];
+    job.out_pwr[e] = sh.pwr[e];
+  }
+  if (tid == 0) *job.out_n = kept;
+}
+
+hipError_t launch_select(const SelectJob* d_jobs, uint32_t n_jobs, hipStream_t stream) {
+  if (n_jobs == 0) return hipSuccess;
+  hipLaunchKernelGGL(select_kernel, dim3(n_jobs), dim3(kSelectThreads), 0, stream, d_jobs);
+  return hipGetLastError();
+}
+
+__global__ void zero_u32_kernel(uint32_t* __restrict__ p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = min((n + 255u) / 256u, 1024u);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace fdas

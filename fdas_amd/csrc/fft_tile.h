@@ -1,0 +1,272 @@
+// Register/LDS FFT building blocks for the fused FT-convolution kernel (gfx950).
+//
+// A tile of S = 16 * NT complex points is transformed by NT = 16 * D threads
+// (D = 4, 8, 16 for S = 1024, 2048, 4096), 16 points per thread, in three
+// register passes separated by two LDS exchanges:
+//
+//   F1  thread b        : DFT-16 over a of x[NT*a + b], twiddle W_S^(b*ka)
+//   F2  thread (d, ka)  : DFT-16 over c of Y[D*c + d][ka], twiddle W_NT^(d*kc)
+//   F3  pairs q = 16*kc + ka : DFT-D over d
+//
+// giving X[ka + 16*kc + 256*kd] = X[q + 256*kd].  The inverse runs the same
+// passes in reverse order with conjugate twiddles, so the spectrum never needs
+// a digit-reversal permutation: the template spectra are stored in the order
+// the F3 pass leaves the tile in (see ftconv_slot_bin).
+//
+// The unnormalised DFT matches the reference's FFT engine (fft_4p.cl:228-319),
+// whose inverse is swap(re,im) o DFT o swap(re,im) (fft_4p.cl:77-88).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fdas {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// a * W with W = conj(w) for the inverse direction (DIR = +1), w for DIR = -1.
+template <int DIR>
+__device__ __forceinline__ float2 ctw(float2 a, float2 w) {
+  if constexpr (DIR < 0) return cmul(a, w);
+  else return cmulc(a, w);
+}
+
+// cos / sin of e*pi/8, e = 0..15.
+struct Oct16 {
+  static constexpr double c[16] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508978,
+                                   0.0, -0.38268343236508978, -0.70710678118654752, -0.92387953251128674,
+                                   -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508978,
+                                   0.0, 0.38268343236508978, 0.70710678118654752, 0.92387953251128674};
+  static constexpr double s[16] = {0.0, 0.38268343236508978, 0.70710678118654752, 0.92387953251128674,
+                                   1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508978,
+                                   0.0, -0.38268343236508978, -0.70710678118654752, -0.92387953251128674,
+                                   -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508978};
+};
+
+// a * exp(DIR * 2*pi*i * E / M) for M in {4, 8, 16}; special angles cost no multiply.
+template <int DIR, int M, int E>
+__device__ __forceinline__ float2 rot(float2 a) {
+  static_assert(M == 4 || M == 8 || M == 16, "rot: M must divide 16");
+  constexpr int e16 = (((E % M) + M) % M) * (16 / M);  // angle in units of pi/8
+  constexpr float r = 0.70710678118654752440f;
+  if constexpr (e16 == 0) {
+    return a;
+  } else if constexpr (e16 == 8) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (e16 == 4) {  // * DIR*i
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+  } else if constexpr (e16 == 12) {  // * -DIR*i
+    return DIR < 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+  } else if constexpr (e16 == 2) {  // (1 + DIR i)/sqrt2
+    return DIR < 0 ? make_float2((a.x + a.y) * r, (a.y - a.x) * r) : make_float2((a.x - a.y) * r, (a.y + a.x) * r);
+  } else if constexpr (e16 == 6) {  // (-1 + DIR i)/sqrt2
+    return DIR < 0 ? make_float2((a.y - a.x) * r, (-a.y - a.x) * r) : make_float2((-a.x - a.y) * r, (a.x - a.y) * r);
+  } else if constexpr (e16 == 10) {  // (-1 - DIR i)/sqrt2
+    return DIR < 0 ? make_float2((-a.x - a.y) * r, (a.x - a.y) * r) : make_float2((a.y - a.x) * r, (-a.y - a.x) * r);
+  } else if constexpr (e16 == 14) {  // (1 - DIR i)/sqrt2
+    return DIR < 0 ? make_float2((a.x - a.y) * r, (a.y + a.x) * r) : make_float2((a.x + a.y) * r, (a.y - a.x) * r);
+  } else {
+    constexpr float cc = static_cast<float>(Oct16::c[e16]);
+    constexpr float ss = static_cast<float>(DIR * Oct16::s[e16]);
+    return make_float2(a.x * cc - a.y * ss, a.x * ss + a.y * cc);
+  }
+}
+
+// In-place DFT-4 of v[o], v[o+st], v[o+2st], v[o+3st]  (outputs in natural order).
+template <int DIR, int O, int ST, int N>
+__device__ __forceinline__ void dft4(float2 (&v)[N]) {
+  const float2 a0 = v[O], a1 = v[O + ST], a2 = v[O + 2 * ST], a3 = v[O + 3 * ST];
+  const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  const float2 t2 = cadd(a1, a3), t3 = rot<DIR, 4, 1>(csub(a1, a3));
+  v[O] = cadd(t0, t2);
+  v[O + 2 * ST] = csub(t0, t2);
+  v[O + ST] = cadd(t1, t3);
+  v[O + 3 * ST] = csub(t1, t3);
+}
+
+// DFT-8 of v[o + st*n], n = 0..7, natural-order output.
+template <int DIR, int O, int ST, int N>
+__device__ __forceinline__ void dft8(float2 (&v)[N]) {
+  float2 e[4] = {v[O], v[O + 2 * ST], v[O + 4 * ST], v[O + 6 * ST]};
+  float2 d[4] = {v[O + ST], v[O + 3 * ST], v[O + 5 * ST], v[O + 7 * ST]};
+  dft4<DIR, 0, 1>(e);
+  dft4<DIR, 0, 1>(d);
+  d[1] = rot<DIR, 8, 1>(d[1]);
+  d[2] = rot<DIR, 8, 2>(d[2]);
+  d[3] = rot<DIR, 8, 3>(d[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[O + k * ST] = cadd(e[k], d[k]);
+    v[O + (k + 4) * ST] = csub(e[k], d[k]);
+  }
+}
+
+template <int DIR, int K1, int N2>
+__device__ __forceinline__ float2 tw16(float2 a) { return rot<DIR, 16, K1 * N2>(a); }
+
+// DFT-16 of v[0..15] (contiguous), natural-order output, as 4 x 4.
+template <int DIR>
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+  // Step 1: DFT-4 over n1 for each n2 (elements n2, 4+n2, 8+n2, 12+n2).
+  dft4<DIR, 0, 4>(v);
+  dft4<DIR, 1, 4>(v);
+  dft4<DIR, 2, 4>(v);
+  dft4<DIR, 3, 4>(v);
+  // Now v[4*k1 + n2] holds B[n2][k1].  Twiddle by W16^(n2*k1).
+  v[5] = tw16<DIR, 1, 1>(v[5]);
+  v[6] = tw16<DIR, 1, 2>(v[6]);
+  v[7] = tw16<DIR, 1, 3>(v[7]);
+  v[9] = tw16<DIR, 2, 1>(v[9]);
+  v[10] = tw16<DIR, 2, 2>(v[10]);
+  v[11] = tw16<DIR, 2, 3>(v[11]);
+  v[13] = tw16<DIR, 3, 1>(v[13]);
+  v[14] = tw16<DIR, 3, 2>(v[14]);
+  v[15] = tw16<DIR, 3, 3>(v[15]);
+  // Step 2: DFT-4 over n2 for each k1: elements 4*k1 .. 4*k1+3 -> X[k1 + 4*k2].
+  dft4<DIR, 0, 1>(v);
+  dft4<DIR, 4, 1>(v);
+  dft4<DIR, 8, 1>(v);
+  dft4<DIR, 12, 1>(v);
+  // v[4*k1 + k2] = X[k1 + 4*k2]: transpose 4x4 in registers (renaming only).
+  float2 t[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = t[i];
+}
+
+// DFT-D over v[O + i], i = 0..D-1 (contiguous group), natural-order output.
+template <int DIR, int D, int O, int N>
+__device__ __forceinline__ void dftD(float2 (&v)[N]) {
+  if constexpr (D == 4) {
+    dft4<DIR, O, 1>(v);
+  } else if constexpr (D == 8) {
+    dft8<DIR, O, 1>(v);
+  } else {
+    static_assert(D == 16, "D must be 4, 8 or 16");
+    float2 t[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t[i] = v[O + i];
+    dft16<DIR>(t);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[O + i] = t[i];
+  }
+}
+
+// Compile-time layout of one S-point tile transform.
+template <int S>
+struct TileFFT {
+  static_assert(S == 1024 || S == 2048 || S == 4096, "tile must be 1024, 2048 or 4096");
+  static constexpr int NT = S / 16;     // threads per transform
+  static constexpr int D = NT / 16;     // radix of the last pass
+  static constexpr int PAIRS = 16 / D;  // (ka, kc) pairs per thread in pass 3
+  static constexpr int RS1 = NT + 2;    // row stride of the [ka][b] exchange
+  static constexpr int RS2 = 256 + 16;  // row stride of the [d][q] exchange
+  static constexpr int L1_ELEMS = 16 * RS1;
+  static constexpr int L2_ELEMS = D * RS2;
+  static constexpr int LDS_ELEMS = L1_ELEMS > L2_ELEMS ? L1_ELEMS : L2_ELEMS;
+
+  // Frequency bin held by thread `tid` in register slot `slot` after pass 3
+  // (slot = i*D + kd for pair q = tid + NT*i).
+  __host__ __device__ static constexpr int slot_bin(int tid, int slot) {
+    return tid + NT * (slot / D) + 256 * (slot % D);
+  }
+
+  // Per-thread twiddles: w1[ka] = W_S^(b*ka) (b = tid), w2[kc] = W_NT^(d*kc) (d = tid/16).
+  float2 w1[16];
+  float2 w2[16];
+
+  __device__ __forceinline__ void load_twiddles(const float2* __restrict__ tw1, const float2* __restrict__ tw2,
+                                                int tid) {
+    w1[0] = make_float2(1.f, 0.f);
+    w2[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+      w1[k] = tw1[k * NT + tid];
+      w2[k] = tw2[k * D + (tid >> 4)];
+    }
+  }
+
+  // Forward transform: x (16 points, x[a] = tile[NT*a + tid]) -> v (16 spectrum
+  // slots, slot_bin order).  Ends with a barrier after the last LDS read.
+  __device__ __forceinline__ void forward(float2 (&v)[16], float2* __restrict__ lds, int tid) const {
+    // F1
+    dft16<-1>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], w1[k]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[k * RS1 + tid] = v[k];
+    __syncthreads();
+    // F2: thread (d, ka) reads Y[D*c + d][ka] for c = 0..15
+    const int ka = tid & 15, d = tid >> 4;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = lds[ka * RS1 + D * c + d];
+    __syncthreads();
+    dft16<-1>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], w2[k]);
+#pragma unroll
+    for (int kc = 0; kc < 16; ++kc) lds[d * RS2 + 16 * kc + ka] = v[kc];
+    __syncthreads();
+    // F3: pairs q = tid + NT*i, DFT-D over d
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+#pragma unroll
+      for (int dd = 0; dd < D; ++dd) v[i * D + dd] = lds[dd * RS2 + tid + NT * i];
+    __syncthreads();
+    apply_dftD<-1>(v);
+  }
+
+  // Inverse transform of spectrum slots v -> v[a] = out[NT*a + tid] (unnormalised).
+  // Uses two LDS buffers: a (pair layout) and b (row layout).  The caller must
+  // ensure no thread still reads `a` from a previous forward pass.
+  __device__ __forceinline__ void inverse(float2 (&v)[16], float2* __restrict__ la, float2* __restrict__ lb,
+                                          int tid) const {
+    apply_dftD<+1>(v);
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+#pragma unroll
+      for (int dd = 0; dd < D; ++dd) la[dd * RS2 + tid + NT * i] = v[i * D + dd];
+    __syncthreads();
+    const int ka = tid & 15, d = tid >> 4;
+#pragma unroll
+    for (int kc = 0; kc < 16; ++kc) v[kc] = la[d * RS2 + 16 * kc + ka];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w2[k]);
+    dft16<+1>(v);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lb[ka * RS1 + D * c + d] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lb[k * RS1 + tid];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w1[k]);
+    dft16<+1>(v);
+  }
+
+ private:
+  template <int DIR>
+  __device__ __forceinline__ static void apply_dftD(float2 (&v)[16]) {
+    if constexpr (PAIRS == 1) {
+      dftD<DIR, D, 0>(v);
+    } else if constexpr (PAIRS == 2) {
+      dftD<DIR, D, 0>(v);
+      dftD<DIR, D, D>(v);
+    } else {
+      dftD<DIR, D, 0>(v);
+      dftD<DIR, D, D>(v);
+      dftD<DIR, D, 2 * D>(v);
+      dftD<DIR, D, 3 * D>(v);
+    }
+  }
+};
+
+}  // namespace fdas

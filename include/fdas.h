@@ -1,0 +1,204 @@
+/*
+ * fdas.h -- C ABI of the MI355X-native FDAS hot path.
+ *
+ * This is the drop-in boundary for the device path of UOA-PARC-SKA/FDAS: each
+ * entry point below replaces one method of the reference's host class
+ * `FDAS` (fdas/src/host/FDAS.h:36-187, implemented in fdas/src/host/FDAS.cpp),
+ * cited per function.  Plain pointers and sizes only; no OpenCL, HIP or torch
+ * types cross the boundary.  Complex values are interleaved float pairs
+ * (re, im), the layout of cl_float2 / std::complex<float> / numpy complex64.
+ *
+ * Conventions (kept from the reference, FDAS.cpp:34-52, FDAS.h:115-181):
+ *  - every call returns a status; on failure the message is written to the
+ *    log callback as "[ERROR] ..." and kept for fdas_last_error();
+ *  - the library owns every device object, the caller owns host arrays;
+ *  - one host thread per context (not thread-safe), like the reference class.
+ */
+#ifndef FDAS_AMD_FDAS_H
+#define FDAS_AMD_FDAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDAS_ABI_VERSION 1
+
+typedef struct fdas_ctx fdas_ctx;
+
+typedef enum fdas_status {
+  FDAS_OK = 0,
+  FDAS_EINVAL = 1,        /* bad argument or size */
+  FDAS_ENOMEM = 2,        /* device or host allocation failed, or HBM budget exceeded */
+  FDAS_EDEVICE = 3,       /* a HIP call failed */
+  FDAS_EUNSUPPORTED = 4,  /* valid request outside what this build or format supports */
+  FDAS_EBUSY = 5,         /* resource in use (e.g. serving queue full) */
+  FDAS_ESTATE = 6,        /* call out of order (e.g. convolution before templates) */
+  FDAS_EOVERFLOW = 7      /* detections exceeded the spill capacity; lists are incomplete */
+} fdas_status;
+
+/* FDAS::FOPPart (FDAS.h:38): t in [-c, 0], [0, c] or [-c, c]. */
+typedef enum fdas_part { FDAS_NEGATIVE = 0, FDAS_POSITIVE = 1, FDAS_ALL = 2 } fdas_part;
+
+/* FDAS::BufferSet (FDAS.h:39). */
+typedef enum fdas_set { FDAS_SET_A = 0, FDAS_SET_B = 1 } fdas_set;
+
+/* Which element count fdas_get_size returns (FDAS.h:78-88). */
+typedef enum fdas_size_kind {
+  FDAS_SIZE_INPUT = 0,       /* N complex bins            (get_input_sz)          */
+  FDAS_SIZE_TILES = 1,       /* n_tiles * S complex       (get_tiles_sz)          */
+  FDAS_SIZE_TEMPLATES = 2,   /* T * S complex             (get_templates_sz)      */
+  FDAS_SIZE_THRESHOLDS = 3,  /* H floats                  (get_thresholds_sz)     */
+  FDAS_SIZE_FOP = 4,         /* T * N floats              (get_fop_sz)            */
+  FDAS_SIZE_CANDIDATES = 5   /* H * cands_per_harmonic    (get_candidate_list_sz) */
+} fdas_size_kind;
+
+/* Config flags. */
+#define FDAS_FLAG_REFERENCE_FFT_ORDER 0x1u /* templates/tiles exchanged in the reference's
+                                              4-parallel FFT order (tmpl-gen.py:145-153) */
+#define FDAS_FLAG_SYNC_PIPELINE 0x2u       /* hold each launch's harmonic summing until the
+                                              next launch or end_pipeline (FDAS.cpp:461-464,549-550) */
+
+/* Run-time replacement of the reference's compile-time GenInfo (gen_info.h:26-74)
+ * plus the arguments of initialise_accelerator (FDAS.h:45-48). */
+typedef struct fdas_config {
+  uint32_t n_bins;             /* N, input_sz of initialise_accelerator; 1 .. 2^22 */
+  uint32_t n_templates;        /* T, odd, <= 255 (85 in the reference) */
+  uint32_t max_taps;           /* longest template, sets the overlap (421) */
+  uint32_t tile;               /* FFT size S: 1024, 2048 or 4096 (2048) */
+  uint32_t n_harmonics;        /* H planes, 1 .. 16 (8) */
+  uint32_t cands_per_harmonic; /* candidates kept per plane, <= 8192 (512 -> 4096-entry lists) */
+  uint32_t spill_per_harmonic; /* detections buffered per plane before the cap (0: default 262144) */
+  int32_t device;              /* HIP device ordinal */
+  uint32_t flags;              /* FDAS_FLAG_* */
+} fdas_config;
+
+/* A candidate in wide form (any T, H, N).  template_num = t in [-c, c]. */
+typedef struct fdas_candidate {
+  uint32_t harmonic;
+  int32_t template_num;
+  uint32_t bin;
+  float power;
+} fdas_candidate;
+
+typedef void (*fdas_log_fn)(int level, const char* message, void* user);
+#define FDAS_LOG_INFO 0
+#define FDAS_LOG_WARN 1
+#define FDAS_LOG_ERROR 2
+
+/* Reference defaults: 85 templates, 421 taps, 2048-point tiles, 8 planes,
+ * 512 candidates per plane (4096-entry lists), gen_info.h:28-72. */
+void fdas_default_config(fdas_config* cfg);
+
+/* FDAS::FDAS(std::ostream&) + initialise_accelerator (FDAS.h:43-48, FDAS.cpp:54-261):
+ * selects the device, fixes N and allocates both buffer sets A and B. */
+fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void* user, fdas_ctx** out);
+void fdas_destroy(fdas_ctx* ctx);
+
+/* FDAS::upload_templates (FDAS.h:50, FDAS.cpp:263-269): T x S complex template
+ * spectra, natural order (or 4-parallel FFT order with FDAS_FLAG_REFERENCE_FFT_ORDER).
+ * Blocking. */
+fdas_status fdas_upload_templates(fdas_ctx* ctx, const float* spectra_c64, int set);
+
+/* Time-domain taps (T x n_taps complex, zero padded) instead of spectra; the
+ * library transforms them to the configured tile size.  No reference counterpart:
+ * lets the tile size differ from 2048 (tmpl-gen.py:139-143 does this offline). */
+fdas_status fdas_upload_template_taps(fdas_ctx* ctx, const float* taps_c64, uint32_t n_taps, int set);
+
+/* FDAS::upload_thresholds (FDAS.h:52, FDAS.cpp:271-280): H floats, used when a
+ * harmonic-summing call passes NULL thresholds. */
+fdas_status fdas_upload_thresholds(fdas_ctx* ctx, const float* thresholds, int set);
+
+/* FDAS::perform_input_tiling (FDAS.h:54, FDAS.cpp:282-328): uploads N complex
+ * bins and materialises the transformed tiles (readable with fdas_retrieve_tiles).
+ * Blocking.  The fused convolution re-tiles on the fly and does not need this
+ * buffer; fdas_launch skips it. */
+fdas_status fdas_perform_input_tiling(fdas_ctx* ctx, const float* spectrum_c64, int set);
+
+/* As fdas_perform_input_tiling, with the spectrum already in device memory
+ * (device-to-device copy, no tiles materialised).  No reference counterpart. */
+fdas_status fdas_set_input_device(fdas_ctx* ctx, const void* d_spectrum_c64, int set);
+
+/* FDAS::perform_ft_convolution (FDAS.h:56, FDAS.cpp:330-405): FOP rows of the part.  Blocking. */
+fdas_status fdas_perform_ft_convolution(fdas_ctx* ctx, fdas_part part, int set);
+
+/* FDAS::perform_harmonic_summing (FDAS.h:58, FDAS.cpp:407-546): harmonic
+ * summing, thresholding and candidate selection over the part.  `thresholds`
+ * (H floats) may be NULL to use the uploaded ones.  Blocking. */
+fdas_status fdas_perform_harmonic_summing(fdas_ctx* ctx, const float* thresholds, fdas_part part, int set);
+
+/* FDAS::launch (FDAS.h:60, FDAS.cpp:548-583): asynchronous upload -> FT
+ * convolution -> harmonic summing -> candidate read-back.  The host arrays
+ * must stay valid until fdas_wait(set); loc/pwr receive
+ * fdas_get_size(FDAS_SIZE_CANDIDATES) entries in the reference layout. */
+fdas_status fdas_launch(fdas_ctx* ctx, const float* spectrum_c64, const float* thresholds, uint32_t* det_loc,
+                        float* det_pwr, fdas_part part, int set);
+
+/* As fdas_launch, with the spectrum already resident in the set's input buffer
+ * (from fdas_perform_input_tiling or fdas_set_input_device): FT convolution ->
+ * harmonic summing -> candidate read-back, no host-to-device copy.  No
+ * reference counterpart; it is the device-resident throughput path. */
+fdas_status fdas_launch_resident(fdas_ctx* ctx, const float* thresholds, uint32_t* det_loc, float* det_pwr,
+                                 fdas_part part, int set);
+
+/* FDAS::wait (FDAS.h:62, FDAS.cpp:585-588). */
+fdas_status fdas_wait(fdas_ctx* ctx, int set);
+
+/* FDAS::end_pipeline (FDAS.h:64, FDAS.cpp:590-594): releases the held
+ * harmonic summing of the other set (FDAS_FLAG_SYNC_PIPELINE). */
+fdas_status fdas_end_pipeline(fdas_ctx* ctx, int set);
+
+/* FDAS::retrieve_tiles / retrieve_FOP / inject_FOP / retrieve_candidates (FDAS.h:66-72,
+ * FDAS.cpp:596-632).  Candidate lists use the reference's 32-bit layout: plane-major,
+ * cands_per_harmonic entries per plane, sorted by (template, bin), padded with
+ * encode(1, c+1, 0) / -1.0f.  Returns FDAS_EUNSUPPORTED when T > 127 or H > 8
+ * (the 7-bit / 3-bit fields of gen_info.h:62-63 cannot hold them). */
+fdas_status fdas_retrieve_tiles(fdas_ctx* ctx, float* tiles_c64, int set);
+fdas_status fdas_retrieve_fop(fdas_ctx* ctx, float* fop, int set);
+fdas_status fdas_inject_fop(fdas_ctx* ctx, const float* fop, int set);
+fdas_status fdas_retrieve_candidates(fdas_ctx* ctx, uint32_t* det_loc, float* det_pwr, int set);
+
+/* Wide candidate lists (any T, H).  Writes up to `capacity` entries, plane-major,
+ * each plane sorted by (template, bin); *n_written = entries written;
+ * n_detected[h] (H entries, may be NULL) = detections above threshold before the cap.
+ * No reference counterpart (the reference format cannot encode T > 127 or H > 8). */
+fdas_status fdas_retrieve_candidates_wide(fdas_ctx* ctx, fdas_candidate* out, uint32_t capacity,
+                                          uint32_t* n_written, uint32_t* n_detected, int set);
+
+/* Harmonic plane HP_k (k = 1..H) of the current FOP, T x N floats, rows outside
+ * the part left untouched.  Debug / parity hook; the hot path never materialises
+ * planes (the reference oracle writes them as hps_ref.npy, hsum.py:334). */
+fdas_status fdas_retrieve_harmonic_plane(fdas_ctx* ctx, uint32_t k, float* plane, fdas_part part, int set);
+
+/* FDAS::print_stats / print_events (FDAS.h:74-76, FDAS.cpp:634-686): per-phase
+ * device durations to the log ("[INFO] ..." and "@@@," CSV lines). */
+void fdas_print_stats(fdas_ctx* ctx, int set, int reset);
+void fdas_print_events(fdas_ctx* ctx, int set);
+
+/* Durations in ms of the last launch on `set`: [0] input, [1] FT convolution,
+ * [2] harmonic summing + selection, [3] candidate read-back, [4] total. */
+fdas_status fdas_get_stage_times(fdas_ctx* ctx, int set, float* ms5);
+
+/* FDAS::get_*_sz (FDAS.h:78-88). */
+uint64_t fdas_get_size(const fdas_ctx* ctx, fdas_size_kind which);
+
+const char* fdas_last_error(const fdas_ctx* ctx);
+const char* fdas_status_string(fdas_status s);
+
+/* Template bank generation (tmpl-gen.py:45-78, 154-159): Fourier response of a
+ * tone drifting by drifts[i] bins, zero padded to max_taps, written as
+ * n x max_taps complex floats.  Fresnel integrals are the library's own
+ * (power series + continued fraction).  Host-only; no device needed. */
+fdas_status fdas_make_drift_bank(const double* drifts, uint32_t n, uint32_t max_taps, float* out_c64);
+
+/* Drifts of the reference bank generator (tmpl-gen.py:52-56, 127-133). */
+fdas_status fdas_reference_drifts(uint32_t n_templates, double max_accel, double unaccel_freq, double t_obs,
+                                  double speed_of_light, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDAS_AMD_FDAS_H */

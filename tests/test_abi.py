@@ -1,0 +1,92 @@
+"""CPU checks of the C ABI library: it loads, exports every declared entry point,
+validates configurations before touching a device, and its host-only template
+generator reproduces the reference bank."""
+
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+import fdas_amd
+from fdas_amd import _native as N
+from oracle import fdas_oracle as O
+
+HEADER = N.LIB_PATH.parents[2] / "include" / "fdas.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fdas_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.load()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+    assert set(names) == set(N.SIGNATURES), "ctypes signatures out of sync with include/fdas.h"
+
+
+def test_default_config_is_reference_geninfo():
+    lib = N.load()
+    cfg = N.FdasConfig()
+    lib.fdas_default_config(C.byref(cfg))
+    # gen_info.h:28-72: 85 templates, 421 taps, 2048-point FFT, 8 planes, 4096 candidates
+    assert (cfg.n_templates, cfg.max_taps, cfg.tile, cfg.n_harmonics) == (85, 421, 2048, 8)
+    assert cfg.cands_per_harmonic * cfg.n_harmonics == 4096
+
+
+@pytest.mark.parametrize("field,value", [("n_bins", 0), ("n_bins", (1 << 22) + 1), ("n_templates", 84),
+                                         ("n_templates", 257), ("tile", 3000), ("max_taps", 2048),
+                                         ("n_harmonics", 0), ("n_harmonics", 17), ("cands_per_harmonic", 0),
+                                         ("cands_per_harmonic", 8193)])
+def test_invalid_config_rejected_before_device(field, value):
+    lib = N.load()
+    cfg = N.FdasConfig()
+    lib.fdas_default_config(C.byref(cfg))
+    cfg.n_bins = 4096
+    setattr(cfg, field, value)
+    msgs = []
+    cb = N.LOG_FN(lambda lvl, m, u: msgs.append((lvl, m.decode())))
+    ctx = C.c_void_p()
+    st = lib.fdas_create(C.byref(cfg), cb, None, C.byref(ctx))
+    assert st == 1  # FDAS_EINVAL
+    assert not ctx
+    assert msgs and msgs[-1][0] == 2 and msgs[-1][1].startswith("[ERROR]")
+
+
+def test_null_arguments():
+    lib = N.load()
+    assert lib.fdas_create(None, N.LOG_FN(0), None, None) == 1
+    assert lib.fdas_get_size(None, 0) == 0
+    assert lib.fdas_last_error(None) == b""
+    assert lib.fdas_status_string(7) == b"candidate spill overflow"
+    assert lib.fdas_make_drift_bank(None, 1, 1, None) == 1
+
+
+def test_python_mirror_reports_failure_as_false():
+    f = fdas_amd.FDAS(log=lambda lvl, msg: None)
+    assert f.initialise_accelerator(4096, n_templates=84) is False
+    assert f.last_status == 1
+
+
+def test_drift_bank_equals_reference_fixture(reference_bank):
+    bank = fdas_amd.make_drift_bank(fdas_amd.reference_drifts(), 421)
+    assert np.array_equal(bank, reference_bank)
+
+
+@pytest.mark.parametrize("n_templates", [85, 169])
+def test_drift_bank_matches_scipy_fresnel(n_templates):
+    from fdas_amd import synth
+    zs = synth.drift_grid(n_templates)
+    ours = fdas_amd.make_drift_bank(zs, 421)
+    ref = O.make_bank(zs, 421, np.complex128)
+    assert np.abs(ours - ref).max() < 1e-6
+
+
+def test_drift_bank_rejects_too_long_template():
+    with pytest.raises(ValueError):
+        fdas_amd.make_drift_bank(np.array([300.0]), 421)  # 600 taps > 421
