@@ -39,7 +39,9 @@ struct DevSet {
   float* plane = nullptr;
   uint32_t* spill_key = nullptr;
   float* spill_pwr = nullptr;
-  uint32_t* counts = nullptr;
+  uint32_t* counts = nullptr;      // [H] detections, then [H] compact counts, then [H][kSelectBins] histograms
+  uint32_t* compact_key = nullptr;
+  float* compact_pwr = nullptr;
   uint32_t* out_key = nullptr;
   float* out_pwr = nullptr;
   uint32_t* out_n = nullptr;
@@ -251,22 +253,29 @@ fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* 
   j.spill_pwr = s.spill_pwr;
   j.counts = s.counts;
   for (uint32_t k = 0; k < H; ++k) j.thr[k] = thr[k];
+  const uint32_t cap = ctx->cfg.cands_per_harmonic, ccap = cap + kSelectPool;
   for (uint32_t p = 0; p < H; ++p) {
     SelectJob& q = s.h_sel[p];
     q.spill_key = s.spill_key + size_t(p) * ctx->spill;
     q.spill_pwr = s.spill_pwr + size_t(p) * ctx->spill;
     q.count = s.counts + p;
     q.spill_cap = ctx->spill;
-    q.cap = ctx->cfg.cands_per_harmonic;
-    q.out_key = s.out_key + size_t(p) * ctx->cfg.cands_per_harmonic;
-    q.out_pwr = s.out_pwr + size_t(p) * ctx->cfg.cands_per_harmonic;
+    q.cap = cap;
+    q.thr = thr[p];
+    q.compact_cap = ccap;
+    q.hist = s.counts + 2 * H + size_t(p) * kSelectBins;
+    q.compact_key = s.compact_key + size_t(p) * ccap;
+    q.compact_pwr = s.compact_pwr + size_t(p) * ccap;
+    q.compact_n = s.counts + H + p;
+    q.out_key = s.out_key + size_t(p) * cap;
+    q.out_pwr = s.out_pwr + size_t(p) * cap;
     q.out_n = s.out_n + p;
   }
   HIPCK(ctx, hipMemcpyAsync(s.d_hsum, s.h_hsum, sizeof(HsumJob), hipMemcpyHostToDevice, s.stream));
   HIPCK(ctx, hipMemcpyAsync(s.d_sel, s.h_sel, sizeof(SelectJob) * H, hipMemcpyHostToDevice, s.stream));
   HIPCK(ctx, hipEventRecord(s.staged_hsum, s.stream));
   s.hsum_pending = true;
-  HIPCK(ctx, launch_zero_u32(s.counts, H, s.stream));
+  HIPCK(ctx, launch_zero_u32(s.counts, H * (2 + kSelectBins), s.stream));
   const uint32_t blocks = (ctx->geo.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock;
   HIPCK(ctx, launch_hsum(s.d_hsum, 1, blocks, H, s.stream));
   HIPCK(ctx, launch_select(s.d_sel, H, s.stream));
@@ -337,6 +346,8 @@ void free_set(DevSet& s) {
   (void)hipFree(s.spill_key);
   (void)hipFree(s.spill_pwr);
   (void)hipFree(s.counts);
+  (void)hipFree(s.compact_key);
+  (void)hipFree(s.compact_pwr);
   (void)hipFree(s.out_key);
   (void)hipFree(s.out_pwr);
   (void)hipFree(s.out_n);
@@ -482,7 +493,9 @@ extern "C" fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void
     if ((st = dev_alloc(ctx, &s.fop, T * N)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.spill_key, H * ctx->spill)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.spill_pwr, H * ctx->spill)) != FDAS_OK) return bail(st);
-    if ((st = dev_alloc(ctx, &s.counts, H)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.counts, H * (2 + kSelectBins))) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.compact_key, H * (cap + kSelectPool))) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.compact_pwr, H * (cap + kSelectPool))) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.out_key, H * cap)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.out_pwr, H * cap)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.out_n, H)) != FDAS_OK) return bail(st);
@@ -508,7 +521,7 @@ extern "C" fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void
     // a FOP that was never written reads as zeros
     if (hipMemset(s.fop, 0, T * N * sizeof(float)) != hipSuccess ||
         hipMemset(s.out_n, 0, H * sizeof(uint32_t)) != hipSuccess ||
-        hipMemset(s.counts, 0, H * sizeof(uint32_t)) != hipSuccess)
+        hipMemset(s.counts, 0, H * (2 + kSelectBins) * sizeof(uint32_t)) != hipSuccess)
       return bail(fail(ctx, FDAS_EDEVICE, "buffer initialisation failed"));
   }
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(ctx, FDAS_EDEVICE, "device synchronisation failed"));

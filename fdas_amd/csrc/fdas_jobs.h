@@ -49,6 +49,12 @@ struct SelectJob {
   const uint32_t* count;      // detections of this plane
   uint32_t spill_cap;
   uint32_t cap;               // entries kept
+  float thr;                  // the plane's threshold: base of the power histogram
+  uint32_t compact_cap;       // capacity of the compact list (cap + boundary pool)
+  uint32_t* hist;             // [kSelectBins] power histogram of the spill (zeroed per run)
+  uint32_t* compact_key;      // [compact_cap] entries at or above the cut bin
+  float* compact_pwr;
+  uint32_t* compact_n;        // zeroed per run
   uint32_t* out_key;          // [cap] sorted by key
   float* out_pwr;
   uint32_t* out_n;            // kept entries

@@ -10,10 +10,14 @@
 namespace fdas {
 
 constexpr int kHsumThreads = 256;
-constexpr int kHsumBinsPerBlock = 2048;   // bins one HSUM workgroup covers (all rows of its part)
+constexpr int kHsumBinsPerThread = 4;
+constexpr int kHsumBinsPerBlock = kHsumThreads * kHsumBinsPerThread;  // bins one HSUM workgroup covers
 constexpr int kHsumStage = 256;           // LDS staging entries per plane before a global flush
 constexpr int kSelectThreads = 1024;
 constexpr uint32_t kSelectMaxCap = 8192;  // largest per-plane cap the in-LDS sort handles
+constexpr uint32_t kSelectPool = 4096;    // boundary-bin entries resolved exactly in LDS
+constexpr uint32_t kSelectBins = 2048;    // power histogram bins (256 per octave above the threshold)
+constexpr uint32_t kSelectSlices = 64;    // workgroups per plane in the histogram / compaction passes
 
 int ftconv_threads(int tile);
 
@@ -29,7 +33,9 @@ hipError_t launch_tiles(int tile, const float2* spectrum, uint32_t n_bins, uint3
 hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, uint32_t max_harmonics,
                        hipStream_t stream);
 
-// Per-plane cap (strongest first) and sort by (row, bin); one workgroup per job.
+// Per-plane cap (strongest first) and sort by (row, bin): power histogram and
+// compaction over the spill lists (kSelectSlices workgroups per plane), then one
+// workgroup per plane resolves the boundary bin exactly and sorts.
 hipError_t launch_select(const SelectJob* d_jobs, uint32_t n_jobs, hipStream_t stream);
 
 // Harmonic plane HP_k over rows [row_begin, row_begin + n_rows) (debug / parity).

@@ -158,18 +158,10 @@ struct HsumStage {
   uint32_t base[kMaxHarmonics];
 };
 
-// Appends the lanes with `hit` set to plane p's LDS stage; entries beyond the
-// stage go straight to the global spill list.
-__device__ __forceinline__ void stage_append(HsumStage& st, const HsumJob& job, uint32_t p, bool hit, uint32_t key,
-                                             float pwr) {
-  const uint64_t mask = __ballot(hit);
-  if (mask == 0) return;
-  const uint32_t lane = lane_id();
-  const uint32_t leader = uint32_t(__builtin_ctzll(mask));
-  uint32_t pos = 0;
-  if (lane == leader) pos = atomicAdd(&st.n[p], uint32_t(__builtin_popcountll(mask)));
-  pos = __shfl(pos, int(leader)) + lanes_below(mask);
-  if (!hit) return;
+// Appends one detection to plane p's LDS stage; entries beyond the stage go
+// straight to the global spill list.
+__device__ __forceinline__ void stage_push(HsumStage& st, const HsumJob& job, uint32_t p, uint32_t key, float pwr) {
+  const uint32_t pos = atomicAdd(&st.n[p], 1u);
   if (pos < uint32_t(kHsumStage)) {
     st.key[p][pos] = key;
     st.pwr[p][pos] = pwr;
@@ -191,6 +183,48 @@ __device__ __forceinline__ uint32_t find_job_hsum(const HsumJob* __restrict__ jo
   return lo;
 }
 
+// Every FOP value one thread needs for one row: its 4 own bins and, for each
+// harmonic k >= 2, the (at most two) sub-harmonic bins (f0 .. f0+3) / k of
+// row trunc(t / k).
+template <int HMAX>
+struct RowLoads {
+  float m[kHsumBinsPerThread];
+  float lo[HMAX];
+  float hi[HMAX];
+};
+
+template <int HMAX>
+__device__ __forceinline__ void hsum_load_row(RowLoads<HMAX>& L, const float* __restrict__ fop, uint32_t n_bins,
+                                              int c, uint32_t r, uint32_t H, uint32_t f0) {
+  const float* __restrict__ row = fop + size_t(r) * n_bins;
+  const uint32_t last = n_bins - 1;
+  if ((n_bins & 3u) == 0 && f0 + 3 < n_bins) {
+    const float4 q = *reinterpret_cast<const float4*>(row + f0);
+    L.m[0] = q.x;
+    L.m[1] = q.y;
+    L.m[2] = q.z;
+    L.m[3] = q.w;
+  } else {
+#pragma unroll
+    for (int b = 0; b < kHsumBinsPerThread; ++b) L.m[b] = row[min(f0 + uint32_t(b), last)];
+  }
+  const int t = int(r) - c;
+  const uint32_t fa = min(f0, last), fb = min(f0 + 3u, last);
+#pragma unroll
+  for (int k = 2; k <= HMAX; ++k) {
+    if (uint32_t(k) <= H) {
+      const float* __restrict__ srow = fop + size_t(c + t / k) * n_bins;
+      L.lo[k - 1] = srow[fa / uint32_t(k)];
+      L.hi[k - 1] = srow[fb / uint32_t(k)];
+    }
+  }
+}
+
+// HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] in FP32, in k order (hsum.py:331);
+// strict HP_k > thr_k (detect.cl.mako:81-83).  One workgroup covers 1024 bins
+// of every row of its part; each thread 4 consecutive bins.  All loads of a row
+// are issued before any compare, and the next row is fetched while the current
+// one is summed.
 template <int HMAX>
 __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __restrict__ jobs, uint32_t n_jobs) {
   __shared__ HsumStage st;
@@ -203,27 +237,45 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   const uint32_t n_bins = job.n_bins;
   const int c = int(job.n_templates / 2);
   const float* __restrict__ fop = job.fop;
-  const uint32_t f_block = (blockIdx.x - job.block_begin) * uint32_t(kHsumBinsPerBlock);
+  const uint32_t f0 = (blockIdx.x - job.block_begin) * uint32_t(kHsumBinsPerBlock) +
+                      uint32_t(tid * kHsumBinsPerThread);
+  const uint32_t r_begin = job.row_begin, r_end = job.row_begin + job.n_rows;
 
-  for (int sub = 0; sub < kHsumBinsPerBlock / kHsumThreads; ++sub) {
-    const uint32_t f = f_block + uint32_t(sub * kHsumThreads + tid);
-    if (f_block + uint32_t(sub * kHsumThreads) >= n_bins) break;  // uniform
-    const bool valid = f < n_bins;
-    const uint32_t fs = valid ? f : 0;
-    for (uint32_t r = job.row_begin; r < job.row_begin + job.n_rows; ++r) {
-      const int t = int(r) - c;
-      float hp = valid ? fop[size_t(r) * n_bins + fs] : 0.f;
-      const uint32_t key = (r << kKeyBinBits) | fs;
-      stage_append(st, job, 0, valid && hp > job.thr[0], key, hp);
+  RowLoads<HMAX> cur, nxt;
+  hsum_load_row<HMAX>(cur, fop, n_bins, c, r_begin, H, f0);
+  for (uint32_t r = r_begin; r < r_end; ++r) {
+    if (r + 1 < r_end) hsum_load_row<HMAX>(nxt, fop, n_bins, c, r + 1, H, f0);
+    const uint32_t fa = min(f0, n_bins - 1);
+    float hp[kHsumBinsPerThread];
 #pragma unroll
-      for (int k = 2; k <= HMAX; ++k) {
-        if (uint32_t(k) <= H) {
-          const int rk = c + t / k;
-          hp += valid ? fop[size_t(rk) * n_bins + fs / uint32_t(k)] : 0.f;
-          stage_append(st, job, uint32_t(k - 1), valid && hp > job.thr[k - 1], key, hp);
+    for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] = cur.m[b];
+#pragma unroll
+    for (int k = 1; k <= HMAX; ++k) {
+      if (uint32_t(k) <= H) {
+        if (k > 1) {
+          const uint32_t qa = fa / uint32_t(k);
+#pragma unroll
+          for (int b = 0; b < kHsumBinsPerThread; ++b)
+            hp[b] += ((f0 + uint32_t(b)) / uint32_t(k) == qa) ? cur.lo[k - 1] : cur.hi[k - 1];
+        }
+        const float thr = job.thr[k - 1];
+        uint32_t hk = 0;
+#pragma unroll
+        for (int b = 0; b < kHsumBinsPerThread; ++b)
+          hk |= uint32_t(f0 + uint32_t(b) < n_bins && hp[b] > thr) << b;
+        if (__ballot(hk != 0)) {
+          while (hk) {
+            const int b = __builtin_ctz(hk);
+            hk &= hk - 1;
+            float v = hp[0];
+#pragma unroll
+            for (int bb = 1; bb < kHsumBinsPerThread; ++bb) v = (b == bb) ? hp[bb] : v;
+            stage_push(st, job, uint32_t(k - 1), (r << kKeyBinBits) | (f0 + uint32_t(b)), v);
+          }
         }
       }
     }
+    cur = nxt;
   }
   __syncthreads();
   if (tid < int(H)) {
@@ -285,8 +337,10 @@ hipError_t launch_hplane(const float* fop, uint32_t n_bins, uint32_t n_templates
 // (row, bin) key first), returned sorted by key.
 // ---------------------------------------------------------------------------
 
-// Exclusive prefix sum over the workgroup (kSelectThreads threads); returns the total in *total.
+// Exclusive prefix sum over a workgroup of THREADS threads; *total gets the sum.
+template <int THREADS>
 __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_waves, uint32_t* total) {
+  constexpr int n_waves = THREADS / 64;
   const int tid = threadIdx.x;
   const int lane = int(lane_id());
   const int wave = tid >> 6;
@@ -299,8 +353,7 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_waves, uint32_t
   if (lane == 63) s_waves[wave] = x;
   __syncthreads();
   if (tid < 64) {
-    const int n_waves = kSelectThreads / 64;
-    uint32_t w = tid < n_waves ? s_waves[tid] : 0u;
+    const uint32_t w = tid < n_waves ? s_waves[tid] : 0u;
     uint32_t ws = w;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -317,9 +370,109 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_waves, uint32_t
   return res;
 }
 
+// Histogram bin of a detection's power: 256 bins per octave above the plane's
+// threshold (bits of non-negative floats are monotonic in the value).
+__device__ __forceinline__ uint32_t threshold_bits(float thr) { return thr > 0.f ? __float_as_uint(thr) : 0u; }
+__device__ __forceinline__ uint32_t power_bin(uint32_t pbits, uint32_t tbits) {
+  return pbits <= tbits ? 0u : min((pbits - tbits) >> 15, kSelectBins - 1);
+}
+
+// Cut bin b*: the highest bin whose suffix count sum_{b >= b*} hist[b] reaches
+// `cap`.  s_out[0] = b*, s_out[1] = entries strictly above b*.
+template <int THREADS>
+__device__ void find_cut(const uint32_t* __restrict__ hist, uint32_t cap, uint32_t* s_waves, uint32_t* s_out) {
+  constexpr int PER = int(kSelectBins) / THREADS;
+  const int tid = threadIdx.x;
+  uint32_t h[PER];
+  uint32_t local = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    h[j] = hist[kSelectBins - 1 - uint32_t(tid * PER + j)];
+    local += h[j];
+  }
+  uint32_t total;
+  const uint32_t excl = block_exclusive_scan<THREADS>(local, s_waves, &total);
+  if (excl < cap && cap <= excl + local) {
+    uint32_t acc = excl;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (acc < cap && cap <= acc + h[j]) {
+        s_out[0] = kSelectBins - 1 - uint32_t(tid * PER + j);
+        s_out[1] = acc;
+      }
+      acc += h[j];
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void cand_hist_kernel(const SelectJob* __restrict__ jobs) {
+  const SelectJob& job = jobs[blockIdx.y];
+  const uint32_t n = min(*job.count, job.spill_cap);
+  if (n <= job.cap) return;
+  __shared__ uint32_t h[kSelectBins];
+  const int tid = threadIdx.x;
+  for (uint32_t i = tid; i < kSelectBins; i += 256) h[i] = 0;
+  __syncthreads();
+  const uint32_t tb = threshold_bits(job.thr);
+  const uint32_t per = (n + kSelectSlices - 1) / kSelectSlices;
+  const uint32_t beg = blockIdx.x * per, end = min(beg + per, n);
+  const float* __restrict__ pw = job.spill_pwr;
+  for (uint32_t e0 = beg; e0 < end; e0 += 4 * 256) {
+    float v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t e = e0 + uint32_t(u * 256 + tid);
+      v[u] = e < end ? pw[e] : -1.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (v[u] >= 0.f) atomicAdd(&h[power_bin(__float_as_uint(v[u]), tb)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < kSelectBins; i += 256)
+    if (h[i]) atomicAdd(&job.hist[i], h[i]);
+}
+
+__global__ __launch_bounds__(256) void cand_compact_kernel(const SelectJob* __restrict__ jobs) {
+  const SelectJob& job = jobs[blockIdx.y];
+  const uint32_t n = min(*job.count, job.spill_cap);
+  if (n <= job.cap) return;
+  __shared__ uint32_t waves[40];
+  __shared__ uint32_t cut[2];
+  find_cut<256>(job.hist, job.cap, waves, cut);
+  const uint32_t bstar = cut[0];
+  const uint32_t tb = threshold_bits(job.thr);
+  const int tid = threadIdx.x;
+  const uint32_t per = (n + kSelectSlices - 1) / kSelectSlices;
+  const uint32_t beg = blockIdx.x * per, end = min(beg + per, n);
+  for (uint32_t e0 = beg; e0 < end; e0 += 256) {
+    const uint32_t e = e0 + uint32_t(tid);
+    float v = -1.f;
+    uint32_t key = 0;
+    if (e < end) {
+      v = job.spill_pwr[e];
+      key = job.spill_key[e];
+    }
+    const bool sel = e < end && power_bin(__float_as_uint(v), tb) >= bstar;
+    const uint64_t mask = __ballot(sel);
+    if (mask == 0) continue;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = uint32_t(__builtin_ctzll(mask));
+    uint32_t pos = 0;
+    if (lane == leader) pos = atomicAdd(job.compact_n, uint32_t(__builtin_popcountll(mask)));
+    pos = __shfl(pos, int(leader)) + lanes_below(mask);
+    if (sel && pos < job.compact_cap) {
+      job.compact_key[pos] = key;
+      job.compact_pwr[pos] = v;
+    }
+  }
+}
+
 struct SelectShared {
   uint32_t key[kSelectMaxCap];
   float pwr[kSelectMaxCap];
+  uint64_t pool[kSelectPool];
   uint32_t hist[2048];
   uint32_t waves[40];
   uint32_t misc[8];
@@ -349,7 +502,7 @@ __device__ void radix_pass(SelectShared& sh, uint32_t n, Get get, uint32_t shift
     h[j] = u < nb ? sh.hist[d] : 0u;
   }
   uint32_t total;
-  const uint32_t excl = block_exclusive_scan(h[0] + h[1], sh.waves, &total);
+  const uint32_t excl = block_exclusive_scan<kSelectThreads>(h[0] + h[1], sh.waves, &total);
   if (excl < rank && rank <= excl + h[0] + h[1]) {
     uint32_t u = uint32_t(2 * tid);
     uint32_t r = rank - excl;
@@ -399,57 +552,127 @@ __device__ void bitonic_sort_by_key(SelectShared& sh, uint32_t n) {
   }
 }
 
-__global__ __launch_bounds__(kSelectThreads) void select_kernel(const SelectJob* __restrict__ jobs) {
-  __shared__ SelectShared sh;
-  const SelectJob& job = jobs[blockIdx.x];
+__device__ void bitonic_sort_u64(uint64_t* a, uint32_t n) {
   const int tid = threadIdx.x;
-  const uint32_t n_det = *job.count;
-  const uint32_t n = min(n_det, job.spill_cap);
+  uint32_t p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  for (uint32_t i = n + tid; i < p2; i += kSelectThreads) a[i] = ~0ull;
+  __syncthreads();
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < p2; i += kSelectThreads) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t x = a[i], y = a[ixj];
+          if ((x > y) == ((i & k) == 0)) {
+            a[i] = y;
+            a[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Exact top-`cap` of the whole spill by radix select on (power desc, key asc).
+// Used when the boundary bin holds more entries than the LDS pool.
+__device__ uint32_t radix_select_spill(SelectShared& sh, const SelectJob& job, uint32_t n) {
+  const int tid = threadIdx.x;
   const uint32_t cap = job.cap;
   const uint32_t* __restrict__ skey = job.spill_key;
   const float* __restrict__ spwr = job.spill_pwr;
+  auto pbits = [&](uint32_t e) { return __float_as_uint(spwr[e]); };
+  uint32_t prefix = 0, mask = 0, rank = cap;
+  radix_pass<true>(sh, n, pbits, 21, 11, prefix, mask, rank);
+  radix_pass<true>(sh, n, pbits, 10, 11, prefix, mask, rank);
+  radix_pass<true>(sh, n, pbits, 0, 10, prefix, mask, rank);
+  const uint32_t vbits = prefix;
+  const uint32_t n_equal = sh.misc[2];
+  // `rank` entries with power == V are kept: the smallest keys among them.
+  uint32_t kmax = 0xffffffffu;
+  if (n_equal > rank) {
+    auto kbits = [&](uint32_t e) { return __float_as_uint(spwr[e]) == vbits ? skey[e] : 0xffffffffu; };
+    uint32_t kp = 0, km = 0, kr = rank;
+    radix_pass<false>(sh, n, kbits, 21, 11, kp, km, kr);
+    radix_pass<false>(sh, n, kbits, 10, 11, kp, km, kr);
+    radix_pass<false>(sh, n, kbits, 0, 10, kp, km, kr);
+    kmax = kp;
+  }
+  if (tid == 0) sh.misc[3] = 0;
+  __syncthreads();
+  for (uint32_t e = tid; e < n; e += kSelectThreads) {
+    const uint32_t pb = __float_as_uint(spwr[e]);
+    const uint32_t key = skey[e];
+    if (pb > vbits || (pb == vbits && key <= kmax)) {
+      const uint32_t pos = atomicAdd(&sh.misc[3], 1u);
+      if (pos < cap) {
+        sh.key[pos] = key;
+        sh.pwr[pos] = spwr[e];
+      }
+    }
+  }
+  __syncthreads();
+  return min(sh.misc[3], cap);
+}
+
+__global__ __launch_bounds__(kSelectThreads) void select_final_kernel(const SelectJob* __restrict__ jobs) {
+  __shared__ SelectShared sh;
+  const SelectJob& job = jobs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const uint32_t n = min(*job.count, job.spill_cap);
+  const uint32_t cap = job.cap;
 
   uint32_t kept;
   if (n <= cap) {
     for (uint32_t e = tid; e < n; e += kSelectThreads) {
-      sh.key[e] = skey[e];
-      sh.pwr[e] = spwr[e];
+      sh.key[e] = job.spill_key[e];
+      sh.pwr[e] = job.spill_pwr[e];
     }
     kept = n;
   } else {
-    // Radix select on the power bits (non-negative floats order like their bits).
-    auto pbits = [&](uint32_t e) { return __float_as_uint(spwr[e]); };
-    uint32_t prefix = 0, mask = 0, rank = cap;
-    radix_pass<true>(sh, n, pbits, 21, 11, prefix, mask, rank);
-    radix_pass<true>(sh, n, pbits, 10, 11, prefix, mask, rank);
-    radix_pass<true>(sh, n, pbits, 0, 10, prefix, mask, rank);
-    const uint32_t vbits = prefix;
-    const uint32_t n_equal = sh.misc[2];
-    // `rank` entries with power == V are kept: the smallest keys among them.
-    uint32_t kmax = 0xffffffffu;
-    if (n_equal > rank) {
-      auto kbits = [&](uint32_t e) { return __float_as_uint(spwr[e]) == vbits ? skey[e] : 0xffffffffu; };
-      uint32_t kp = 0, km = 0, kr = rank;
-      radix_pass<false>(sh, n, kbits, 21, 11, kp, km, kr);
-      radix_pass<false>(sh, n, kbits, 10, 11, kp, km, kr);
-      radix_pass<false>(sh, n, kbits, 0, 10, kp, km, kr);
-      kmax = kp;
-    }
-    if (tid == 0) sh.misc[3] = 0;
+    find_cut<kSelectThreads>(job.hist, cap, sh.waves, sh.misc);
+    const uint32_t bstar = sh.misc[0], above = sh.misc[1];
+    const uint32_t m = *job.compact_n;
+    const uint32_t pool_n = job.hist[bstar];
     __syncthreads();
-    for (uint32_t e = tid; e < n; e += kSelectThreads) {
-      const uint32_t pb = __float_as_uint(spwr[e]);
-      const uint32_t key = skey[e];
-      if (pb > vbits || (pb == vbits && key <= kmax)) {
-        const uint32_t pos = atomicAdd(&sh.misc[3], 1u);
-        if (pos < cap) {
-          sh.key[pos] = key;
-          sh.pwr[pos] = spwr[e];
+    if (m <= job.compact_cap && pool_n <= kSelectPool && above < cap) {
+      const uint32_t tb = threshold_bits(job.thr);
+      if (tid == 0) {
+        sh.misc[3] = 0;
+        sh.misc[4] = 0;
+      }
+      __syncthreads();
+      for (uint32_t e = tid; e < m; e += kSelectThreads) {
+        const float v = job.compact_pwr[e];
+        const uint32_t key = job.compact_key[e];
+        const uint32_t pb = __float_as_uint(v);
+        const uint32_t bin = power_bin(pb, tb);
+        if (bin > bstar) {
+          const uint32_t pos = atomicAdd(&sh.misc[3], 1u);
+          if (pos < cap) {
+            sh.key[pos] = key;
+            sh.pwr[pos] = v;
+          }
+        } else if (bin == bstar) {
+          const uint32_t q = atomicAdd(&sh.misc[4], 1u);
+          if (q < kSelectPool) sh.pool[q] = (uint64_t(~pb) << 32) | key;
         }
       }
+      __syncthreads();
+      const uint32_t n_pool = min(sh.misc[4], kSelectPool);
+      bitonic_sort_u64(sh.pool, n_pool);
+      const uint32_t take = cap - above;
+      for (uint32_t i = tid; i < take && i < n_pool; i += kSelectThreads) {
+        const uint64_t v = sh.pool[i];
+        sh.key[above + i] = uint32_t(v);
+        sh.pwr[above + i] = __uint_as_float(~uint32_t(v >> 32));
+      }
+      __syncthreads();
+      kept = cap;
+    } else {
+      kept = radix_select_spill(sh, job, n);
     }
-    __syncthreads();
-    kept = min(sh.misc[3], cap);
   }
   __syncthreads();
   bitonic_sort_by_key(sh, kept);
@@ -462,7 +685,9 @@ __global__ __launch_bounds__(kSelectThreads) void select_kernel(const SelectJob*
 
 hipError_t launch_select(const SelectJob* d_jobs, uint32_t n_jobs, hipStream_t stream) {
   if (n_jobs == 0) return hipSuccess;
-  hipLaunchKernelGGL(select_kernel, dim3(n_jobs), dim3(kSelectThreads), 0, stream, d_jobs);
+  hipLaunchKernelGGL(cand_hist_kernel, dim3(kSelectSlices, n_jobs), dim3(256), 0, stream, d_jobs);
+  hipLaunchKernelGGL(cand_compact_kernel, dim3(kSelectSlices, n_jobs), dim3(256), 0, stream, d_jobs);
+  hipLaunchKernelGGL(select_final_kernel, dim3(n_jobs), dim3(kSelectThreads), 0, stream, d_jobs);
   return hipGetLastError();
 }
 
