@@ -59,7 +59,8 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
 #pragma unroll
     for (int a = 0; a < 16; ++a) {
       const int idx = base + NT * a + tid;
-      xf[a] = (idx >= 0 && idx < int(n_bins)) ? x[idx] : make_float2(0.f, 0.f);
+      xf[a] = make_float2(0.f, 0.f);
+      if (idx >= 0 && idx < int(n_bins)) xf[a] = gld2(x, size_t(idx));
     }
     fft.forward(xf, la, tid);
 
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
       const float2* __restrict__ h = hbase + size_t(r) * S + tid;
       float2 v[16];
 #pragma unroll
-      for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], h[s * NT]);
+      for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], gld2(h, size_t(s * NT)));
       fft.inverse(v, la, lb, tid);
       float* __restrict__ out = job->fop + size_t(r) * n_bins;
 #pragma unroll
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
         const uint32_t s = uint32_t(NT * a + tid);
         if (s >= overlap) {
           const uint32_t f = out0 + s - overlap;
-          if (f < n_bins) out[f] = v[a].x * v[a].x + v[a].y * v[a].y;
+          if (f < n_bins) gst(out, f, v[a].x * v[a].x + v[a].y * v[a].y);
         }
       }
     }
@@ -166,10 +167,10 @@ __device__ __forceinline__ void stage_push(HsumStage& st, const HsumJob& job, ui
     st.key[p][pos] = key;
     st.pwr[p][pos] = pwr;
   } else {
-    const uint32_t g = atomicAdd(&job.counts[p], 1u);
+    const uint32_t g = gatomic_add(job.counts + p, 1u);
     if (g < job.spill_cap) {
-      job.spill_key[size_t(p) * job.spill_cap + g] = key;
-      job.spill_pwr[size_t(p) * job.spill_cap + g] = pwr;
+      gst(job.spill_key, size_t(p) * job.spill_cap + g, key);
+      gst(job.spill_pwr, size_t(p) * job.spill_cap + g, pwr);
     }
   }
 }
@@ -199,14 +200,14 @@ __device__ __forceinline__ void hsum_load_row(RowLoads<HMAX>& L, const float* __
   const float* __restrict__ row = fop + size_t(r) * n_bins;
   const uint32_t last = n_bins - 1;
   if ((n_bins & 3u) == 0 && f0 + 3 < n_bins) {
-    const float4 q = *reinterpret_cast<const float4*>(row + f0);
+    const fdas_vf4 q = gld4(row, f0);
     L.m[0] = q.x;
     L.m[1] = q.y;
     L.m[2] = q.z;
     L.m[3] = q.w;
   } else {
 #pragma unroll
-    for (int b = 0; b < kHsumBinsPerThread; ++b) L.m[b] = row[min(f0 + uint32_t(b), last)];
+    for (int b = 0; b < kHsumBinsPerThread; ++b) L.m[b] = gld(row, min(f0 + uint32_t(b), last));
   }
   const int t = int(r) - c;
   const uint32_t fa = min(f0, last), fb = min(f0 + 3u, last);
@@ -214,8 +215,8 @@ __device__ __forceinline__ void hsum_load_row(RowLoads<HMAX>& L, const float* __
   for (int k = 2; k <= HMAX; ++k) {
     if (uint32_t(k) <= H) {
       const float* __restrict__ srow = fop + size_t(c + t / k) * n_bins;
-      L.lo[k - 1] = srow[fa / uint32_t(k)];
-      L.hi[k - 1] = srow[fb / uint32_t(k)];
+      L.lo[k - 1] = gld(srow, fa / uint32_t(k));
+      L.hi[k - 1] = gld(srow, fb / uint32_t(k));
     }
   }
 }
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   __syncthreads();
   if (tid < int(H)) {
     const uint32_t n = min(st.n[tid], uint32_t(kHsumStage));
-    st.base[tid] = n ? atomicAdd(&job.counts[tid], n) : 0u;
+    st.base[tid] = n ? gatomic_add(job.counts + tid, n) : 0u;
   }
   __syncthreads();
   for (uint32_t p = 0; p < H; ++p) {
@@ -289,8 +290,8 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
     for (uint32_t i = tid; i < n; i += kHsumThreads) {
       const uint32_t g = base + i;
       if (g < job.spill_cap) {
-        job.spill_key[size_t(p) * job.spill_cap + g] = st.key[p][i];
-        job.spill_pwr[size_t(p) * job.spill_cap + g] = st.pwr[p][i];
+        gst(job.spill_key, size_t(p) * job.spill_cap + g, st.key[p][i]);
+        gst(job.spill_pwr, size_t(p) * job.spill_cap + g, st.pwr[p][i]);
       }
     }
   }
@@ -387,7 +388,7 @@ __device__ void find_cut(const uint32_t* __restrict__ hist, uint32_t cap, uint32
   uint32_t local = 0;
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
-    h[j] = hist[kSelectBins - 1 - uint32_t(tid * PER + j)];
+    h[j] = gld(hist, kSelectBins - 1 - uint32_t(tid * PER + j));
     local += h[j];
   }
   uint32_t total;
@@ -408,7 +409,7 @@ __device__ void find_cut(const uint32_t* __restrict__ hist, uint32_t cap, uint32
 
 __global__ __launch_bounds__(256) void cand_hist_kernel(const SelectJob* __restrict__ jobs) {
   const SelectJob& job = jobs[blockIdx.y];
-  const uint32_t n = min(*job.count, job.spill_cap);
+  const uint32_t n = min(gld(job.count, 0), job.spill_cap);
   if (n <= job.cap) return;
   __shared__ uint32_t h[kSelectBins];
   const int tid = threadIdx.x;
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(256) void cand_hist_kernel(const SelectJob* __restr
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = e0 + uint32_t(u * 256 + tid);
-      v[u] = e < end ? pw[e] : -1.f;
+      v[u] = e < end ? gld(pw, e) : -1.f;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -431,12 +432,12 @@ __global__ __launch_bounds__(256) void cand_hist_kernel(const SelectJob* __restr
   }
   __syncthreads();
   for (uint32_t i = tid; i < kSelectBins; i += 256)
-    if (h[i]) atomicAdd(&job.hist[i], h[i]);
+    if (h[i]) gatomic_add(job.hist + i, h[i]);
 }
 
 __global__ __launch_bounds__(256) void cand_compact_kernel(const SelectJob* __restrict__ jobs) {
   const SelectJob& job = jobs[blockIdx.y];
-  const uint32_t n = min(*job.count, job.spill_cap);
+  const uint32_t n = min(gld(job.count, 0), job.spill_cap);
   if (n <= job.cap) return;
   __shared__ uint32_t waves[40];
   __shared__ uint32_t cut[2];
@@ -451,8 +452,8 @@ __global__ __launch_bounds__(256) void cand_compact_kernel(const SelectJob* __re
     float v = -1.f;
     uint32_t key = 0;
     if (e < end) {
-      v = job.spill_pwr[e];
-      key = job.spill_key[e];
+      v = gld(job.spill_pwr, e);
+      key = gld(job.spill_key, e);
     }
     const bool sel = e < end && power_bin(__float_as_uint(v), tb) >= bstar;
     const uint64_t mask = __ballot(sel);
@@ -460,11 +461,11 @@ __global__ __launch_bounds__(256) void cand_compact_kernel(const SelectJob* __re
     const uint32_t lane = lane_id();
     const uint32_t leader = uint32_t(__builtin_ctzll(mask));
     uint32_t pos = 0;
-    if (lane == leader) pos = atomicAdd(job.compact_n, uint32_t(__builtin_popcountll(mask)));
+    if (lane == leader) pos = gatomic_add(job.compact_n, uint32_t(__builtin_popcountll(mask)));
     pos = __shfl(pos, int(leader)) + lanes_below(mask);
     if (sel && pos < job.compact_cap) {
-      job.compact_key[pos] = key;
-      job.compact_pwr[pos] = v;
+      gst(job.compact_key, pos, key);
+      gst(job.compact_pwr, pos, v);
     }
   }
 }
@@ -582,7 +583,7 @@ __device__ uint32_t radix_select_spill(SelectShared& sh, const SelectJob& job, u
   const uint32_t cap = job.cap;
   const uint32_t* __restrict__ skey = job.spill_key;
   const float* __restrict__ spwr = job.spill_pwr;
-  auto pbits = [&](uint32_t e) { return __float_as_uint(spwr[e]); };
+  auto pbits = [&](uint32_t e) { return __float_as_uint(gld(spwr, e)); };
   uint32_t prefix = 0, mask = 0, rank = cap;
   radix_pass<true>(sh, n, pbits, 21, 11, prefix, mask, rank);
   radix_pass<true>(sh, n, pbits, 10, 11, prefix, mask, rank);
@@ -592,7 +593,7 @@ __device__ uint32_t radix_select_spill(SelectShared& sh, const SelectJob& job, u
   // `rank` entries with power == V are kept: the smallest keys among them.
   uint32_t kmax = 0xffffffffu;
   if (n_equal > rank) {
-    auto kbits = [&](uint32_t e) { return __float_as_uint(spwr[e]) == vbits ? skey[e] : 0xffffffffu; };
+    auto kbits = [&](uint32_t e) { return __float_as_uint(gld(spwr, e)) == vbits ? gld(skey, e) : 0xffffffffu; };
     uint32_t kp = 0, km = 0, kr = rank;
     radix_pass<false>(sh, n, kbits, 21, 11, kp, km, kr);
     radix_pass<false>(sh, n, kbits, 10, 11, kp, km, kr);
@@ -602,13 +603,14 @@ __device__ uint32_t radix_select_spill(SelectShared& sh, const SelectJob& job, u
   if (tid == 0) sh.misc[3] = 0;
   __syncthreads();
   for (uint32_t e = tid; e < n; e += kSelectThreads) {
-    const uint32_t pb = __float_as_uint(spwr[e]);
-    const uint32_t key = skey[e];
+    const float pv = gld(spwr, e);
+    const uint32_t pb = __float_as_uint(pv);
+    const uint32_t key = gld(skey, e);
     if (pb > vbits || (pb == vbits && key <= kmax)) {
       const uint32_t pos = atomicAdd(&sh.misc[3], 1u);
       if (pos < cap) {
         sh.key[pos] = key;
-        sh.pwr[pos] = spwr[e];
+        sh.pwr[pos] = pv;
       }
     }
   }
@@ -620,21 +622,21 @@ __global__ __launch_bounds__(kSelectThreads) void select_final_kernel(const Sele
   __shared__ SelectShared sh;
   const SelectJob& job = jobs[blockIdx.x];
   const int tid = threadIdx.x;
-  const uint32_t n = min(*job.count, job.spill_cap);
+  const uint32_t n = min(gld(job.count, 0), job.spill_cap);
   const uint32_t cap = job.cap;
 
   uint32_t kept;
   if (n <= cap) {
     for (uint32_t e = tid; e < n; e += kSelectThreads) {
-      sh.key[e] = job.spill_key[e];
-      sh.pwr[e] = job.spill_pwr[e];
+      sh.key[e] = gld(job.spill_key, e);
+      sh.pwr[e] = gld(job.spill_pwr, e);
     }
     kept = n;
   } else {
     find_cut<kSelectThreads>(job.hist, cap, sh.waves, sh.misc);
     const uint32_t bstar = sh.misc[0], above = sh.misc[1];
-    const uint32_t m = *job.compact_n;
-    const uint32_t pool_n = job.hist[bstar];
+    const uint32_t m = gld(job.compact_n, 0);
+    const uint32_t pool_n = gld(job.hist, bstar);
     __syncthreads();
     if (m <= job.compact_cap && pool_n <= kSelectPool && above < cap) {
       const uint32_t tb = threshold_bits(job.thr);
@@ -644,8 +646,8 @@ __global__ __launch_bounds__(kSelectThreads) void select_final_kernel(const Sele
       }
       __syncthreads();
       for (uint32_t e = tid; e < m; e += kSelectThreads) {
-        const float v = job.compact_pwr[e];
-        const uint32_t key = job.compact_key[e];
+        const float v = gld(job.compact_pwr, e);
+        const uint32_t key = gld(job.compact_key, e);
         const uint32_t pb = __float_as_uint(v);
         const uint32_t bin = power_bin(pb, tb);
         if (bin > bstar) {
@@ -677,10 +679,10 @@ __global__ __launch_bounds__(kSelectThreads) void select_final_kernel(const Sele
   __syncthreads();
   bitonic_sort_by_key(sh, kept);
   for (uint32_t e = tid; e < kept; e += kSelectThreads) {
-    job.out_key[e] = sh.key[e];
-    job.out_pwr[e] = sh.pwr[e];
+    gst(job.out_key, e, sh.key[e]);
+    gst(job.out_pwr, e, sh.pwr[e]);
   }
-  if (tid == 0) *job.out_n = kept;
+  if (tid == 0) gst(job.out_n, 0, kept);
 }
 
 hipError_t launch_select(const SelectJob* d_jobs, uint32_t n_jobs, hipStream_t stream) {

@@ -21,6 +21,35 @@
 
 namespace fdas {
 
+// Global-memory access helpers.  Pointers read from a job descriptor are
+// generic to the compiler, which then emits FLAT instructions; FLAT memory ops
+// also count against the LDS wait counter, so every LDS wait would drain the
+// outstanding HBM traffic too.  These helpers address global memory (address
+// space 1) explicitly, so the loads and stores stay global_load / global_store.
+#define FDAS_GLOBAL __attribute__((address_space(1)))
+typedef float fdas_vf2 __attribute__((ext_vector_type(2)));
+typedef float fdas_vf4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, size_t i) {
+  return ((const FDAS_GLOBAL T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, size_t i, T v) {
+  ((FDAS_GLOBAL T*)p)[i] = v;
+}
+__device__ __forceinline__ float2 gld2(const float2* p, size_t i) {
+  const fdas_vf2 v = ((const FDAS_GLOBAL fdas_vf2*)p)[i];
+  return make_float2(v.x, v.y);
+}
+// Four consecutive floats p[i .. i+3]; p + i must be 16-byte aligned.
+__device__ __forceinline__ fdas_vf4 gld4(const float* p, size_t i) {
+  return *(const FDAS_GLOBAL fdas_vf4*)(p + i);
+}
+__device__ __forceinline__ uint32_t gatomic_add(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add((FDAS_GLOBAL uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
