@@ -30,7 +30,8 @@ hipError_t launch_tiles(int tile, const float2* spectrum, uint32_t n_bins, uint3
                         uint32_t n_tiles, const float2* tw1, const float2* tw2, float2* out, hipStream_t stream);
 
 // Harmonic summing + thresholding + compaction into per-plane spill lists.
-hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, uint32_t max_harmonics,
+// All jobs of one launch share n_harmonics; vec requires every job's n_bins % 4 == 0.
+hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, uint32_t n_harmonics, bool vec,
                        hipStream_t stream);
 
 // Per-plane cap (strongest first) and sort by (row, bin): power histogram and

@@ -186,21 +186,25 @@ __device__ __forceinline__ uint32_t find_job_hsum(const HsumJob* __restrict__ jo
 
 // Every FOP value one thread needs for one row: its 4 own bins and, for each
 // harmonic k >= 2, the (at most two) sub-harmonic bins (f0 .. f0+3) / k of
-// row trunc(t / k).
-template <int HMAX>
+// row trunc(t / k).  f0 is a multiple of 4, so (f0 .. f0+3) / 2 spans two bins.
+template <int H>
 struct RowLoads {
   float m[kHsumBinsPerThread];
-  float lo[HMAX];
-  float hi[HMAX];
+  float lo[H];
+  float hi[H];
 };
 
-template <int HMAX>
-__device__ __forceinline__ void hsum_load_row(RowLoads<HMAX>& L, const float* __restrict__ fop, uint32_t n_bins,
-                                              int c, uint32_t r, uint32_t H, uint32_t f0) {
+// Branch-free: H is a compile-time constant and out-of-range bins are clamped,
+// so the compiler can count the loads in flight exactly (a load issued under a
+// condition makes it drain every outstanding load at the next use).
+template <int H, bool VEC>
+__device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __restrict__ fop, uint32_t n_bins, int c,
+                                              uint32_t r, uint32_t f0) {
   const float* __restrict__ row = fop + size_t(r) * n_bins;
   const uint32_t last = n_bins - 1;
-  if ((n_bins & 3u) == 0 && f0 + 3 < n_bins) {
-    const fdas_vf4 q = gld4(row, f0);
+  if constexpr (VEC) {
+    // n_bins % 4 == 0: the whole quad is either in range or entirely past the end
+    const fdas_vf4 q = gld4(row, min(f0, n_bins - 4));
     L.m[0] = q.x;
     L.m[1] = q.y;
     L.m[2] = q.z;
@@ -212,26 +216,55 @@ __device__ __forceinline__ void hsum_load_row(RowLoads<HMAX>& L, const float* __
   const int t = int(r) - c;
   const uint32_t fa = min(f0, last), fb = min(f0 + 3u, last);
 #pragma unroll
-  for (int k = 2; k <= HMAX; ++k) {
-    if (uint32_t(k) <= H) {
-      const float* __restrict__ srow = fop + size_t(c + t / k) * n_bins;
-      L.lo[k - 1] = gld(srow, fa / uint32_t(k));
-      L.hi[k - 1] = gld(srow, fb / uint32_t(k));
+  for (int k = 2; k <= H; ++k) {
+    const float* __restrict__ srow = fop + size_t(c + t / k) * n_bins;
+    L.lo[k - 1] = gld(srow, fa / uint32_t(k));
+    L.hi[k - 1] = gld(srow, fb / uint32_t(k));
+  }
+}
+
+template <int H>
+__device__ __forceinline__ void hsum_row(HsumStage& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
+                                         uint32_t f0, uint32_t n_bins, const float (&thr)[H]) {
+  const uint32_t fa = min(f0, n_bins - 1);
+  float hp[kHsumBinsPerThread];
+#pragma unroll
+  for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] = L.m[b];
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    if (k > 1) {
+      const uint32_t qa = fa / uint32_t(k);
+#pragma unroll
+      for (int b = 0; b < kHsumBinsPerThread; ++b)
+        hp[b] += ((f0 + uint32_t(b)) / uint32_t(k) == qa) ? L.lo[k - 1] : L.hi[k - 1];
+    }
+    uint32_t hk = 0;
+#pragma unroll
+    for (int b = 0; b < kHsumBinsPerThread; ++b)
+      hk |= uint32_t(f0 + uint32_t(b) < n_bins && hp[b] > thr[k - 1]) << b;
+    if (__ballot(hk != 0)) {
+      while (hk) {
+        const int b = __builtin_ctz(hk);
+        hk &= hk - 1;
+        float v = hp[0];
+#pragma unroll
+        for (int bb = 1; bb < kHsumBinsPerThread; ++bb) v = (b == bb) ? hp[bb] : v;
+        stage_push(st, job, uint32_t(k - 1), (r << kKeyBinBits) | (f0 + uint32_t(b)), v);
+      }
     }
   }
 }
 
 // HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] in FP32, in k order (hsum.py:331);
 // strict HP_k > thr_k (detect.cl.mako:81-83).  One workgroup covers 1024 bins
-// of every row of its part; each thread 4 consecutive bins.  All loads of a row
-// are issued before any compare, and the next row is fetched while the current
-// one is summed.
-template <int HMAX>
+// of every row of its part, each thread 4 consecutive bins.  Rows are processed
+// in pairs with two register buffers: the loads of row r+1 are in flight while
+// row r is summed and thresholded.
+template <int H, bool VEC>
 __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __restrict__ jobs, uint32_t n_jobs) {
   __shared__ HsumStage st;
   const HsumJob& job = jobs[find_job_hsum(jobs, n_jobs, blockIdx.x)];
   const int tid = threadIdx.x;
-  const uint32_t H = job.n_harmonics;
   if (tid < int(kMaxHarmonics)) st.n[tid] = 0;
   __syncthreads();
 
@@ -240,51 +273,28 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   const float* __restrict__ fop = job.fop;
   const uint32_t f0 = (blockIdx.x - job.block_begin) * uint32_t(kHsumBinsPerBlock) +
                       uint32_t(tid * kHsumBinsPerThread);
-  const uint32_t r_begin = job.row_begin, r_end = job.row_begin + job.n_rows;
+  const uint32_t r_begin = job.row_begin, r_last = job.row_begin + job.n_rows - 1;
+  float thr[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) thr[k] = job.thr[k];
 
-  RowLoads<HMAX> cur, nxt;
-  hsum_load_row<HMAX>(cur, fop, n_bins, c, r_begin, H, f0);
-  for (uint32_t r = r_begin; r < r_end; ++r) {
-    if (r + 1 < r_end) hsum_load_row<HMAX>(nxt, fop, n_bins, c, r + 1, H, f0);
-    const uint32_t fa = min(f0, n_bins - 1);
-    float hp[kHsumBinsPerThread];
-#pragma unroll
-    for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] = cur.m[b];
-#pragma unroll
-    for (int k = 1; k <= HMAX; ++k) {
-      if (uint32_t(k) <= H) {
-        if (k > 1) {
-          const uint32_t qa = fa / uint32_t(k);
-#pragma unroll
-          for (int b = 0; b < kHsumBinsPerThread; ++b)
-            hp[b] += ((f0 + uint32_t(b)) / uint32_t(k) == qa) ? cur.lo[k - 1] : cur.hi[k - 1];
-        }
-        const float thr = job.thr[k - 1];
-        uint32_t hk = 0;
-#pragma unroll
-        for (int b = 0; b < kHsumBinsPerThread; ++b)
-          hk |= uint32_t(f0 + uint32_t(b) < n_bins && hp[b] > thr) << b;
-        if (__ballot(hk != 0)) {
-          while (hk) {
-            const int b = __builtin_ctz(hk);
-            hk &= hk - 1;
-            float v = hp[0];
-#pragma unroll
-            for (int bb = 1; bb < kHsumBinsPerThread; ++bb) v = (b == bb) ? hp[bb] : v;
-            stage_push(st, job, uint32_t(k - 1), (r << kKeyBinBits) | (f0 + uint32_t(b)), v);
-          }
-        }
-      }
-    }
-    cur = nxt;
+  RowLoads<H> a, b;
+  hsum_load_row<H, VEC>(a, fop, n_bins, c, r_begin, f0);
+  for (uint32_t r = r_begin;; r += 2) {
+    hsum_load_row<H, VEC>(b, fop, n_bins, c, min(r + 1, r_last), f0);
+    hsum_row<H>(st, job, a, r, f0, n_bins, thr);
+    if (r + 1 > r_last) break;
+    hsum_load_row<H, VEC>(a, fop, n_bins, c, min(r + 2, r_last), f0);
+    hsum_row<H>(st, job, b, r + 1, f0, n_bins, thr);
+    if (r + 2 > r_last) break;
   }
   __syncthreads();
-  if (tid < int(H)) {
+  if (tid < H) {
     const uint32_t n = min(st.n[tid], uint32_t(kHsumStage));
     st.base[tid] = n ? gatomic_add(job.counts + tid, n) : 0u;
   }
   __syncthreads();
-  for (uint32_t p = 0; p < H; ++p) {
+  for (uint32_t p = 0; p < uint32_t(H); ++p) {
     const uint32_t n = min(st.n[p], uint32_t(kHsumStage));
     const uint32_t base = st.base[p];
     for (uint32_t i = tid; i < n; i += kHsumThreads) {
@@ -297,15 +307,36 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   }
 }
 
-hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, uint32_t max_harmonics,
-                       hipStream_t stream) {
-  if (n_blocks == 0) return hipSuccess;
-  if (max_harmonics <= 8)
-    hipLaunchKernelGGL(hsum_kernel<8>, dim3(n_blocks), dim3(kHsumThreads), 0, stream, d_jobs, n_jobs);
-  else if (max_harmonics <= 16)
-    hipLaunchKernelGGL(hsum_kernel<16>, dim3(n_blocks), dim3(kHsumThreads), 0, stream, d_jobs, n_jobs);
+template <int H>
+static void launch_hsum_h(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, bool vec, hipStream_t stream) {
+  if (vec)
+    hipLaunchKernelGGL((hsum_kernel<H, true>), dim3(n_blocks), dim3(kHsumThreads), 0, stream, d_jobs, n_jobs);
   else
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL((hsum_kernel<H, false>), dim3(n_blocks), dim3(kHsumThreads), 0, stream, d_jobs, n_jobs);
+}
+
+hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, uint32_t n_harmonics,
+                       bool vec, hipStream_t stream) {
+  if (n_blocks == 0) return hipSuccess;
+  switch (n_harmonics) {
+    case 1: launch_hsum_h<1>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 2: launch_hsum_h<2>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 3: launch_hsum_h<3>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 4: launch_hsum_h<4>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 5: launch_hsum_h<5>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 6: launch_hsum_h<6>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 7: launch_hsum_h<7>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 8: launch_hsum_h<8>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 9: launch_hsum_h<9>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 10: launch_hsum_h<10>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 11: launch_hsum_h<11>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 12: launch_hsum_h<12>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 13: launch_hsum_h<13>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 14: launch_hsum_h<14>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 15: launch_hsum_h<15>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    case 16: launch_hsum_h<16>(d_jobs, n_jobs, n_blocks, vec, stream); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
