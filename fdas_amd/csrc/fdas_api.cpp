@@ -249,6 +249,8 @@ fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* 
   j.n_harmonics = H;
   j.spill_cap = ctx->spill;
   j.block_begin = 0;
+  j.n_row_chunks = (nr + kHsumRowsPerBlock - 1) / kHsumRowsPerBlock;
+  j.rows_per_block = (nr + j.n_row_chunks - 1) / j.n_row_chunks;
   j.spill_key = s.spill_key;
   j.spill_pwr = s.spill_pwr;
   j.counts = s.counts;
@@ -276,7 +278,7 @@ fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* 
   HIPCK(ctx, hipEventRecord(s.staged_hsum, s.stream));
   s.hsum_pending = true;
   HIPCK(ctx, launch_zero_u32(s.counts, H * (2 + kSelectBins), s.stream));
-  const uint32_t blocks = (ctx->geo.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock;
+  const uint32_t blocks = (ctx->geo.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
   HIPCK(ctx, launch_hsum(s.d_hsum, 1, blocks, H, (ctx->geo.n_bins & 3u) == 0, s.stream));
   HIPCK(ctx, launch_select(s.d_sel, H, s.stream));
   return FDAS_OK;

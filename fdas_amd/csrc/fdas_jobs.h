@@ -35,7 +35,9 @@ struct HsumJob {
   uint32_t n_rows;
   uint32_t n_harmonics;
   uint32_t spill_cap;     // entries per plane in the spill lists
-  uint32_t block_begin;   // exclusive prefix of bin blocks over jobs
+  uint32_t block_begin;   // exclusive prefix of workgroups over jobs
+  uint32_t rows_per_block;  // rows one workgroup sums (the part is split into row chunks)
+  uint32_t n_row_chunks;    // ceil(n_rows / rows_per_block); workgroups = bin blocks * n_row_chunks
   uint32_t pad_;
   uint32_t* spill_key;    // [H][spill_cap] row << 22 | bin
   float* spill_pwr;       // [H][spill_cap]

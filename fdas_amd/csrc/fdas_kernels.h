@@ -12,7 +12,8 @@ namespace fdas {
 constexpr int kHsumThreads = 256;
 constexpr int kHsumBinsPerThread = 4;
 constexpr int kHsumBinsPerBlock = kHsumThreads * kHsumBinsPerThread;  // bins one HSUM workgroup covers
-constexpr int kHsumStage = 256;           // LDS staging entries per plane before a global flush
+constexpr uint32_t kHsumRowsPerBlock = 11;  // target rows per HSUM workgroup (bounds the dense-hit tail)
+constexpr int kHsumWaveStage = 64;        // LDS staging entries per plane per wave before a global flush
 constexpr int kSelectThreads = 1024;
 constexpr uint32_t kSelectMaxCap = 8192;  // largest per-plane cap the in-LDS sort handles
 constexpr uint32_t kSelectPool = 4096;    // boundary-bin entries resolved exactly in LDS

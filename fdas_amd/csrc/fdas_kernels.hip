@@ -152,26 +152,26 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
+// Per-wave detection staging: each wave owns kHsumWaveStage LDS slots per
+// plane and a wave-uniform fill count.  A full stage is flushed by the wave
+// alone (one global atomic for up to 64 entries); what is left at the end is
+// flushed with one atomic per plane per workgroup.
 struct HsumStage {
-  uint32_t key[kMaxHarmonics][kHsumStage];
-  float pwr[kMaxHarmonics][kHsumStage];
-  uint32_t n[kMaxHarmonics];
-  uint32_t base[kMaxHarmonics];
+  uint32_t key[kHsumThreads / 64][kMaxHarmonics][kHsumWaveStage];
+  float pwr[kHsumThreads / 64][kMaxHarmonics][kHsumWaveStage];
+  uint32_t n[kHsumThreads / 64][kMaxHarmonics];
+  uint32_t base[kHsumThreads / 64][kMaxHarmonics];
 };
 
-// Appends one detection to plane p's LDS stage; entries beyond the stage go
-// straight to the global spill list.
-__device__ __forceinline__ void stage_push(HsumStage& st, const HsumJob& job, uint32_t p, uint32_t key, float pwr) {
-  const uint32_t pos = atomicAdd(&st.n[p], 1u);
-  if (pos < uint32_t(kHsumStage)) {
-    st.key[p][pos] = key;
-    st.pwr[p][pos] = pwr;
-  } else {
-    const uint32_t g = gatomic_add(job.counts + p, 1u);
-    if (g < job.spill_cap) {
-      gst(job.spill_key, size_t(p) * job.spill_cap + g, key);
-      gst(job.spill_pwr, size_t(p) * job.spill_cap + g, pwr);
-    }
+// Writes the wave's n staged entries of plane p to the global spill list.
+__device__ __forceinline__ void wave_flush(HsumStage& st, const HsumJob& job, uint32_t wave, uint32_t p, uint32_t n) {
+  const uint32_t lane = lane_id();
+  uint32_t g = 0;
+  if (lane == 0) g = gatomic_add(job.counts + p, n);
+  g = __shfl(g, 0);
+  if (lane < n && g + lane < job.spill_cap) {
+    gst(job.spill_key, size_t(p) * job.spill_cap + g + lane, st.key[wave][p][lane]);
+    gst(job.spill_pwr, size_t(p) * job.spill_cap + g + lane, st.pwr[wave][p][lane]);
   }
 }
 
@@ -225,8 +225,10 @@ __device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __res
 
 template <int H>
 __device__ __forceinline__ void hsum_row(HsumStage& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
-                                         uint32_t f0, uint32_t n_bins, const float (&thr)[H]) {
+                                         uint32_t f0, uint32_t n_bins, const float (&thr)[H], uint32_t (&wn)[H],
+                                         uint32_t wave) {
   const uint32_t fa = min(f0, n_bins - 1);
+  const uint32_t lane = lane_id();
   float hp[kHsumBinsPerThread];
 #pragma unroll
   for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] = L.m[b];
@@ -238,21 +240,32 @@ __device__ __forceinline__ void hsum_row(HsumStage& st, const HsumJob& job, cons
       for (int b = 0; b < kHsumBinsPerThread; ++b)
         hp[b] += ((f0 + uint32_t(b)) / uint32_t(k) == qa) ? L.lo[k - 1] : L.hi[k - 1];
     }
-    uint32_t hk = 0;
+    bool hit[kHsumBinsPerThread];
+    bool any = false;
 #pragma unroll
-    for (int b = 0; b < kHsumBinsPerThread; ++b)
-      hk |= uint32_t(f0 + uint32_t(b) < n_bins && hp[b] > thr[k - 1]) << b;
-    if (__ballot(hk != 0)) {
-      while (hk) {
-        const int b = __builtin_ctz(hk);
-        hk &= hk - 1;
-        float v = hp[0];
+    for (int b = 0; b < kHsumBinsPerThread; ++b) {
+      hit[b] = f0 + uint32_t(b) < n_bins && hp[b] > thr[k - 1];
+      any |= hit[b];
+    }
+    if (__ballot(any) == 0) continue;
 #pragma unroll
-        for (int bb = 1; bb < kHsumBinsPerThread; ++bb) v = (b == bb) ? hp[bb] : v;
-        stage_push(st, job, uint32_t(k - 1), (r << kKeyBinBits) | (f0 + uint32_t(b)), v);
+    for (int b = 0; b < kHsumBinsPerThread; ++b) {
+      const uint64_t mask = __ballot(hit[b]);
+      if (mask == 0) continue;
+      const uint32_t cnt = uint32_t(__builtin_popcountll(mask));
+      if (wn[k - 1] + cnt > uint32_t(kHsumWaveStage)) {
+        wave_flush(st, job, wave, uint32_t(k - 1), wn[k - 1]);
+        wn[k - 1] = 0;
       }
+      if (hit[b]) {
+        const uint32_t pos = wn[k - 1] + lanes_below(mask);
+        st.key[wave][k - 1][pos] = (r << kKeyBinBits) | (f0 + uint32_t(b));
+        st.pwr[wave][k - 1][pos] = hp[b];
+      }
+      wn[k - 1] += cnt;
     }
   }
+  (void)lane;
 }
 
 // HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] in FP32, in k order (hsum.py:331);
@@ -265,44 +278,57 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   __shared__ HsumStage st;
   const HsumJob& job = jobs[find_job_hsum(jobs, n_jobs, blockIdx.x)];
   const int tid = threadIdx.x;
-  if (tid < int(kMaxHarmonics)) st.n[tid] = 0;
-  __syncthreads();
+  const uint32_t wave = uint32_t(tid) >> 6;
 
   const uint32_t n_bins = job.n_bins;
   const int c = int(job.n_templates / 2);
   const float* __restrict__ fop = job.fop;
-  const uint32_t f0 = (blockIdx.x - job.block_begin) * uint32_t(kHsumBinsPerBlock) +
-                      uint32_t(tid * kHsumBinsPerThread);
-  const uint32_t r_begin = job.row_begin, r_last = job.row_begin + job.n_rows - 1;
+  const uint32_t local = blockIdx.x - job.block_begin;
+  const uint32_t chunk = local % job.n_row_chunks;  // row chunks of one bin block run side by side
+  const uint32_t f0 = (local / job.n_row_chunks) * uint32_t(kHsumBinsPerBlock) + uint32_t(tid * kHsumBinsPerThread);
+  const uint32_t r_begin = job.row_begin + chunk * job.rows_per_block;
+  const uint32_t r_last = min(r_begin + job.rows_per_block, job.row_begin + job.n_rows) - 1;
   float thr[H];
+  uint32_t wn[H];
 #pragma unroll
-  for (int k = 0; k < H; ++k) thr[k] = job.thr[k];
+  for (int k = 0; k < H; ++k) {
+    thr[k] = job.thr[k];
+    wn[k] = 0;
+  }
 
   RowLoads<H> a, b;
   hsum_load_row<H, VEC>(a, fop, n_bins, c, r_begin, f0);
   for (uint32_t r = r_begin;; r += 2) {
     hsum_load_row<H, VEC>(b, fop, n_bins, c, min(r + 1, r_last), f0);
-    hsum_row<H>(st, job, a, r, f0, n_bins, thr);
+    hsum_row<H>(st, job, a, r, f0, n_bins, thr, wn, wave);
     if (r + 1 > r_last) break;
     hsum_load_row<H, VEC>(a, fop, n_bins, c, min(r + 2, r_last), f0);
-    hsum_row<H>(st, job, b, r + 1, f0, n_bins, thr);
+    hsum_row<H>(st, job, b, r + 1, f0, n_bins, thr, wn, wave);
     if (r + 2 > r_last) break;
+  }
+  // Flush what is left: one global atomic per plane for the whole workgroup.
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) st.n[wave][k] = wn[k];
   }
   __syncthreads();
   if (tid < H) {
-    const uint32_t n = min(st.n[tid], uint32_t(kHsumStage));
-    st.base[tid] = n ? gatomic_add(job.counts + tid, n) : 0u;
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < uint32_t(kHsumThreads / 64); ++w) {
+      st.base[w][tid] = total;
+      total += st.n[w][tid];
+    }
+    const uint32_t g = total ? gatomic_add(job.counts + tid, total) : 0u;
+    for (uint32_t w = 0; w < uint32_t(kHsumThreads / 64); ++w) st.base[w][tid] += g;
   }
   __syncthreads();
-  for (uint32_t p = 0; p < uint32_t(H); ++p) {
-    const uint32_t n = min(st.n[p], uint32_t(kHsumStage));
-    const uint32_t base = st.base[p];
-    for (uint32_t i = tid; i < n; i += kHsumThreads) {
-      const uint32_t g = base + i;
-      if (g < job.spill_cap) {
-        gst(job.spill_key, size_t(p) * job.spill_cap + g, st.key[p][i]);
-        gst(job.spill_pwr, size_t(p) * job.spill_cap + g, st.pwr[p][i]);
-      }
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const uint32_t g = st.base[wave][k] + lane;
+    if (lane < wn[k] && g < job.spill_cap) {
+      gst(job.spill_key, size_t(k) * job.spill_cap + g, st.key[wave][k][lane]);
+      gst(job.spill_pwr, size_t(k) * job.spill_cap + g, st.pwr[wave][k][lane]);
     }
   }
 }
