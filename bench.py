@@ -56,17 +56,28 @@ def fft_flops(n_bins, n_templates, tile, overlap):
     return n_tiles * per + n_tiles * n_templates * (per + 6 * tile + 3 * payload)
 
 
-def cpu_baseline(x, bank, thr, cap):
-    """FP64 oracle restatement of ft_conv.py + hsum.py on one full trial (single thread)."""
-    from oracle import fdas_oracle as O
-    t0 = time.perf_counter()
-    fop = O.fop_fft(x, bank)
-    sel, _ = O.detect_streaming(fop, len(thr), thr)
-    O.select(sel, cap)
-    dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 4), "unit": "DM-trials/s", "cores": 1, "kind": "port",
-            "sample": f"1 full trial (N=2^21, 85 templates, 8 harmonics) through the FP64 NumPy oracle "
-                      f"(overlap-save FFT convolution + harmonic planes + selection), {dt:.1f} s, 1 thread"}
+def cpu_baseline(x, bank, thr, cap, budget_s=10.0):
+    """The oracle's C restatement of ft_conv.py + hsum.py (FP32, OpenMP, own FFT) on full trials.
+
+    Runs whole N = 2^21 trials back to back until ~budget_s of CPU time has passed (at least 2).
+    """
+    from oracle import cpu
+    n_threads = cpu.threads()
+    cpu.fop(x[:4096], bank)  # warm-up (page-in, thread pool)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 2 or (time.perf_counter() - t_start < budget_s and len(times) < 50):
+        t0 = time.perf_counter()
+        fop = cpu.fop(x, bank)
+        cpu.hsum(fop, thr, len(thr), cap)
+        times.append(time.perf_counter() - t0)
+        del fop
+    med = float(np.median(times))
+    return {"value": round(1.0 / med, 4), "unit": "DM-trials/s", "cores": n_threads, "kind": "port",
+            "sample": f"{len(times)} full trials (N=2^21, 85 templates x 421 taps, 8 harmonics, 4096/plane) through "
+                      f"oracle/fdas_cpu.c (FP32 overlap-save FFT convolution + harmonic summing + selection), "
+                      f"median {med:.3f} s per trial on {n_threads} OpenMP threads",
+            "ms_per_trial": round(med * 1000.0, 2)}
 
 
 def main():

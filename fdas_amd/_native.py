@@ -74,6 +74,9 @@ SIGNATURES = {
     "fdas_print_events": (None, [_P, _I]),
     "fdas_get_stage_times": (_I, [_P, _I, _F]),
     "fdas_get_size": (C.c_uint64, [_P, _I]),
+    "fdas_device_count": (_I, []),
+    "fdas_device_info": (_I, [_I, C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]),
+    "fdas_platform_version": (C.c_char_p, []),
     "fdas_last_error": (C.c_char_p, [_P]),
     "fdas_status_string": (C.c_char_p, [_I]),
     "fdas_make_drift_bank": (_I, [_D, _u32, _u32, _F]),

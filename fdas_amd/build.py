@@ -54,5 +54,28 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+FLOW_TEST_SRC = ROOT / "tests" / "cpp" / "fdas_flow_test.cpp"
+FLOW_TEST = LIB_DIR / "fdas_flow_test"
+
+
+def build_flow_test(verbose: bool = False) -> Path:
+    """C++ driver of the reference's test flows, linked against libfdas_amd.so (host code only)."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(FLOW_TEST_SRC),
+           f"-L{LIB_DIR}", "-lfdas_amd", "-Wl,-rpath,$ORIGIN", "-o", str(FLOW_TEST)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed ({res.returncode}):\n{res.stdout}{res.stderr}")
+    return FLOW_TEST
+
+
+def build_all(verbose: bool = False):
+    lib = build_library(verbose)
+    test = build_flow_test(verbose)
+    return lib, test
+
+
 if __name__ == "__main__":
-    print(build_library(verbose=True))
+    print(build_all(verbose=True))

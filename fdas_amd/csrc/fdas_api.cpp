@@ -999,3 +999,37 @@ extern "C" void fdas_print_events(fdas_ctx* ctx, int set) {
                 double(t[1]), double(t[2]), double(t[3]), double(t[4]));
   emit(ctx, FDAS_LOG_INFO, buf);
 }
+
+extern "C" int fdas_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+extern "C" fdas_status fdas_device_info(int device, char* name, size_t len, char* arch, size_t arch_len) {
+  const int n = fdas_device_count();
+  if (device < 0 || device >= n) return FDAS_EINVAL;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    (void)hipGetLastError();
+    return FDAS_EDEVICE;
+  }
+  if (name && len) {
+    std::snprintf(name, len, "%s", prop.name[0] ? prop.name : prop.gcnArchName);
+  }
+  if (arch && arch_len) std::snprintf(arch, arch_len, "%s", prop.gcnArchName);
+  return FDAS_OK;
+}
+
+extern "C" const char* fdas_platform_version(void) {
+  static char buf[64] = {0};
+  if (!buf[0]) {
+    int v = 0;
+    if (hipRuntimeGetVersion(&v) != hipSuccess) (void)hipGetLastError();
+    std::snprintf(buf, sizeof(buf), "HIP %d.%d.%d", v / 10000000, (v / 100000) % 100, v % 100000);
+  }
+  return buf;
+}

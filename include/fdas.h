@@ -181,6 +181,14 @@ void fdas_print_events(fdas_ctx* ctx, int set);
  * [2] harmonic summing + selection, [3] candidate read-back, [4] total. */
 fdas_status fdas_get_stage_times(fdas_ctx* ctx, int set, float* ms5);
 
+/* Device enumeration for the platform / device selectors of
+ * initialise_accelerator (FDAS.h:45-47, FDAS.cpp:65-115).  fdas_device_info
+ * writes the device name into `name` (NUL-terminated, at most `len` bytes) and
+ * returns FDAS_EINVAL for an ordinal outside [0, fdas_device_count()). */
+int fdas_device_count(void);
+fdas_status fdas_device_info(int device, char* name, size_t len, char* arch, size_t arch_len);
+const char* fdas_platform_version(void);
+
 /* FDAS::get_*_sz (FDAS.h:78-88). */
 uint64_t fdas_get_size(const fdas_ctx* ctx, fdas_size_kind which);
 
