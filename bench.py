@@ -83,8 +83,12 @@ def cpu_baseline(x, bank, thr, cap, budget_s=10.0):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=40, help="timed DM trials")
     ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--in-flight", type=int, default=2, choices=(1, 2),
+                    help="trials in flight in the timed region (buffer sets A/B)")
+    ap.add_argument("--isolated-steps", type=int, default=10,
+                    help="extra one-at-a-time trials for uncontended kernel durations (0: skip)")
     ap.add_argument("--plain-trials", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=2048)
@@ -118,7 +122,8 @@ def main():
     pwr = {ab: np.zeros(n_list, np.float32) for ab in (FDAS.A, FDAS.B)}
     part = FDAS.AllAccelerations
 
-    def run(n_steps, record, depth=2):
+    def run(n_steps, depth, record=None):
+        """n_steps trials, `depth` in flight on sets A/B; per-trial stage times appended to record."""
         in_flight = []
         for i in range(n_steps):
             ab = i % 2
@@ -134,51 +139,68 @@ def main():
             if record is not None:
                 record.append(f.stage_times(prev))
 
-    forms = {}
-    for depth in (1, 2):
-        log(f"depth {depth}: warmup {args.warmup} steps")
-        run(args.warmup, None, depth)
-        stages = []
-        log(f"depth {depth}: timing {args.steps} steps")
+    isolated = None
+    if args.isolated_steps > 0:
+        run(2, 1)
+        rec = []
         t0 = time.perf_counter()
-        run(args.steps, stages, depth)
-        t1 = time.perf_counter()
-        ms = (t1 - t0) * 1000.0 / args.steps
-        st = np.array(stages)
-        forms[depth] = (ms, float(np.mean(st[:, 1])), float(np.mean(st[:, 2])))
-        log(f"resident depth {depth}: {ms:.3f} ms/trial ({1000.0 / ms:.1f} trials/s); ftc {forms[depth][1]:.3f} ms, "
-            f"hsum+select {forms[depth][2]:.3f} ms")
-    best = min(forms, key=lambda d: forms[d][0])
-    ms_per_step = forms[best][0]
+        run(args.isolated_steps, 1, rec)
+        ms = (time.perf_counter() - t0) * 1000.0 / args.isolated_steps
+        st = np.array(rec)
+        isolated = {"ms_per_trial": round(ms, 4), "trials_per_s": round(1000.0 / ms, 2),
+                    "ftconv_ms": round(float(st[:, 1].mean()), 4), "hsum_ms": round(float(st[:, 2].mean()), 4),
+                    "select_ms": round(float(st[:, 3].mean()), 4)}
+        log(f"isolated (1 in flight): {isolated}")
+
+    log(f"warmup {args.warmup} trials, {args.in_flight} in flight")
+    run(args.warmup, args.in_flight)
+    rec = []
+    log(f"timing {args.steps} trials")
+    t0 = time.perf_counter()
+    run(args.steps, args.in_flight, rec)
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = 1000.0 / ms_per_step
-    # kernel durations from the serial form: one kernel on the device at a time
-    ftc_ms, hsum_ms = forms[1][1], forms[1][2]
+    st = np.array(rec)
+    ftc_ms, hsum_ms, sel_ms = (float(st[:, i].mean()) for i in (1, 2, 3))
+    log(f"timed: {ms_per_step:.3f} ms/trial ({value:.1f} trials/s); kernel means ftconv {ftc_ms:.3f} ms, "
+        f"hsum {hsum_ms:.3f} ms, select {sel_ms:.3f} ms")
 
     # Plain GPU form: one trial per launch, host upload, read-back and synchronisation each time.
-    lat = []
-    for i in range(args.plain_trials + 2):
-        t = time.perf_counter()
-        assert f.launch(x, thr, loc[FDAS.A], pwr[FDAS.A], part, FDAS.A), f.last_error
-        assert f.wait(FDAS.A), f.last_error
-        if i >= 2:
-            lat.append((time.perf_counter() - t) * 1000.0)
-    lat = np.array(lat)
-    plain = {"value": round(float(1000.0 / lat.mean()), 2), "unit": "DM-trials/s",
-             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
-    log(f"plain: {plain}")
+    plain = None
+    if args.plain_trials > 0:
+        lat = []
+        for i in range(args.plain_trials + 2):
+            t = time.perf_counter()
+            assert f.launch(x, thr, loc[FDAS.A], pwr[FDAS.A], part, FDAS.A), f.last_error
+            assert f.wait(FDAS.A), f.last_error
+            if i >= 2:
+                lat.append((time.perf_counter() - t) * 1000.0)
+        lat = np.array(lat)
+        plain = {"value": round(float(1000.0 / lat.mean()), 2), "unit": "DM-trials/s",
+                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                 "form": "host spectrum upload + FT conv + HSUM + selection + read-back + sync, one trial per launch"}
+        log(f"plain: {plain}")
 
     _, _, _, _, det = f.retrieve_candidates_wide(FDAS.A)
     n_cands = int(np.minimum(det, cap).sum())
-    geo_overlap = 420
-    b_ftc = ftc_bytes(w.n_bins, w.n_templates, args.tile)
-    flops = fft_flops(w.n_bins, w.n_templates, args.tile, geo_overlap)
-    achieved = b_ftc / (ftc_ms * 1e-3) / 1e9
+    kernels = {
+        "ftconv_kernel<2048>": {"ms": ftc_ms, "bytes": ftc_bytes(w.n_bins, w.n_templates, args.tile)},
+        "hsum_kernel<8,true>": {"ms": hsum_ms, "bytes": hsum_bytes(w.n_bins, w.n_templates, n_cands)},
+    }
+    dominant = max(kernels, key=lambda k: kernels[k]["ms"])
+    kd = kernels[dominant]
+    achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "ftconv_kernel<2048>", "kernel_ms": round(ftc_ms, 4),
-                "fft_tflops": round(flops / (ftc_ms * 1e-3) / 1e12, 2), "fp32_peak_tflops": FP32_PEAK_TFLOPS,
-                "hsum_ms": round(hsum_ms, 4),
-                "hsum_gbs": round(hsum_bytes(w.n_bins, w.n_templates, n_cands) / (hsum_ms * 1e-3) / 1e9, 1)}
+                "kernel": dominant, "kernel_ms": round(kd["ms"], 4), "algorithmic_bytes": int(kd["bytes"]),
+                "kernels": {k: {"ms": round(v["ms"], 4), "algorithmic_bytes": int(v["bytes"]),
+                                "gbs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in kernels.items()},
+                "ftconv_fft_tflops": round(fft_flops(w.n_bins, w.n_templates, args.tile, 420) / (ftc_ms * 1e-3) / 1e12,
+                                           2),
+                "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS,
+                "select_ms": round(sel_ms, 4), "detections_per_plane": [int(v) for v in det]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -202,10 +224,9 @@ def main():
         "data": "synthetic (seeded N(0,1) series + drifting 8-harmonic tone; 85-template drift bank z=2(j-42))",
         "config": {"workload": "single-trial FDAS, N=2^21 bins, 85 templates x 421 taps, FFT 2048 overlap-save, "
                                "8 harmonics, 4096 candidates/harmonic, spectra resident",
-                   "in_flight": best,
-                   "ms_per_trial_by_in_flight": {str(d): round(v[0], 4) for d, v in forms.items()},
                    "n_bins": w.n_bins, "n_templates": w.n_templates, "tile": args.tile,
-                   "n_harmonics": w.n_harmonics, "cands_per_harmonic": cap},
+                   "n_harmonics": w.n_harmonics, "cands_per_harmonic": cap, "in_flight": args.in_flight},
+        "isolated_form": isolated,
         "plain_gpu_form": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -29,7 +29,9 @@ namespace {
 constexpr uint32_t kDefaultSpill = 262144;
 constexpr double kPi = 3.14159265358979323846264338327950288;
 
-enum Ev { EV_START = 0, EV_INPUT, EV_FTC, EV_HSUM, EV_XFER, EV_COUNT };
+// Events per launch: START, INPUT (spectrum in place), FTC_K (FT-convolution kernel start), FTC (its end),
+// HSUM_K0 / HSUM_K1 (harmonic-summing kernel start / end), HSUM (selection end), XFER (lists read back).
+enum Ev { EV_START = 0, EV_INPUT, EV_FTC_K, EV_FTC, EV_HSUM_K0, EV_HSUM_K1, EV_HSUM, EV_XFER, EV_COUNT };
 
 struct DevSet {
   float2* input = nullptr;
@@ -229,6 +231,7 @@ fdas_status enqueue_ftc(fdas_ctx* ctx, DevSet& s, fdas_part part) {
   HIPCK(ctx, hipMemcpyAsync(s.d_ftc, s.h_ftc, sizeof(FtcJob), hipMemcpyHostToDevice, s.stream));
   HIPCK(ctx, hipEventRecord(s.staged_ftc, s.stream));
   s.ftc_pending = true;
+  if ((st = record(ctx, s, EV_FTC_K)) != FDAS_OK) return st;
   HIPCK(ctx, launch_ftconv(int(ctx->cfg.tile), s.d_ftc, 1, n_items, ctx->tw1, ctx->tw2, n_items, s.stream));
   return FDAS_OK;
 }
@@ -279,7 +282,9 @@ fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* 
   s.hsum_pending = true;
   HIPCK(ctx, launch_zero_u32(s.counts, H * (2 + kSelectBins), s.stream));
   const uint32_t blocks = (ctx->geo.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
+  if ((st = record(ctx, s, EV_HSUM_K0)) != FDAS_OK) return st;
   HIPCK(ctx, launch_hsum(s.d_hsum, 1, blocks, H, (ctx->geo.n_bins & 3u) == 0, s.stream));
+  if ((st = record(ctx, s, EV_HSUM_K1)) != FDAS_OK) return st;
   HIPCK(ctx, launch_select(s.d_sel, H, s.stream));
   return FDAS_OK;
 }
@@ -700,7 +705,7 @@ extern "C" fdas_status fdas_perform_ft_convolution(fdas_ctx* ctx, fdas_part part
   s.fop_valid = true;
   s.cands_valid = false;
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, s.ev[EV_INPUT], s.ev[EV_FTC]);
+  (void)hipEventElapsedTime(&ms, s.ev[EV_FTC_K], s.ev[EV_FTC]);
   info(ctx, "FT convolution and IFFT duration: %.3f ms", double(ms));
   return FDAS_OK;
 }
@@ -729,7 +734,7 @@ extern "C" fdas_status fdas_perform_harmonic_summing(fdas_ctx* ctx, const float*
   HIPCK(ctx, hipEventSynchronize(s.ev[EV_XFER]));
   s.cands_valid = true;
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, s.ev[EV_FTC], s.ev[EV_HSUM]);
+  (void)hipEventElapsedTime(&ms, s.ev[EV_HSUM_K0], s.ev[EV_HSUM]);
   info(ctx, "Harmonic summing duration: %.3f ms", double(ms));
   return overflow_status(ctx, s);
 }
@@ -953,15 +958,16 @@ static float elapsed(const DevSet& s, int a, int b) {
   return ms;
 }
 
-extern "C" fdas_status fdas_get_stage_times(fdas_ctx* ctx, int set, float* ms5) {
+extern "C" fdas_status fdas_get_stage_times(fdas_ctx* ctx, int set, float* ms6) {
   CHECK_SET(ctx, set);
-  if (!ms5) return fail(ctx, FDAS_EINVAL, "null output");
+  if (!ms6) return fail(ctx, FDAS_EINVAL, "null output");
   const DevSet& s = ctx->set[set];
-  ms5[0] = elapsed(s, EV_START, EV_INPUT);
-  ms5[1] = elapsed(s, EV_INPUT, EV_FTC);
-  ms5[2] = elapsed(s, EV_FTC, EV_HSUM);
-  ms5[3] = elapsed(s, EV_HSUM, EV_XFER);
-  ms5[4] = elapsed(s, EV_START, EV_XFER);
+  ms6[0] = elapsed(s, EV_START, EV_INPUT);
+  ms6[1] = elapsed(s, EV_FTC_K, EV_FTC);
+  ms6[2] = elapsed(s, EV_HSUM_K0, EV_HSUM_K1);
+  ms6[3] = elapsed(s, EV_HSUM_K1, EV_HSUM);
+  ms6[4] = elapsed(s, EV_HSUM, EV_XFER);
+  ms6[5] = elapsed(s, EV_START, EV_XFER);
   return FDAS_OK;
 }
 
@@ -982,8 +988,8 @@ extern "C" void fdas_print_stats(fdas_ctx* ctx, int set, int reset) {
   info(ctx, "Launch  #%u%c complete @ %.3f ms II ~ %.3f ms", ctx->pipeline_launches, set == FDAS_SET_A ? 'A' : 'B',
        double(acc), double(acc - ctx->pipeline_accum));
   info(ctx, "  Preparation duration: %.3f ms", double(elapsed(s, EV_START, EV_INPUT)));
-  info(ctx, "  FT convolution and IFFT duration: %.3f ms", double(elapsed(s, EV_INPUT, EV_FTC)));
-  info(ctx, "  Harmonic summing duration: %.3f ms", double(elapsed(s, EV_FTC, EV_HSUM)));
+  info(ctx, "  FT convolution and IFFT duration: %.3f ms", double(elapsed(s, EV_FTC_K, EV_FTC)));
+  info(ctx, "  Harmonic summing duration: %.3f ms", double(elapsed(s, EV_HSUM_K0, EV_HSUM)));
   info(ctx, "  Total duration: %.3f ms", double(elapsed(s, EV_START, EV_XFER)));
   ctx->pipeline_accum = acc;
 }
@@ -995,8 +1001,9 @@ extern "C" void fdas_print_events(fdas_ctx* ctx, int set) {
   for (int e = 0; e < EV_COUNT; ++e)
     if (s.ev_recorded[e]) (void)hipEventElapsedTime(&t[e], ctx->epoch, s.ev[e]);
   char buf[256];
-  std::snprintf(buf, sizeof(buf), "@@@,%c,%.6f,%.6f,%.6f,%.6f,%.6f", set == FDAS_SET_A ? 'A' : 'B', double(t[0]),
-                double(t[1]), double(t[2]), double(t[3]), double(t[4]));
+  std::snprintf(buf, sizeof(buf), "@@@,%c,%.6f,%.6f,%.6f,%.6f,%.6f,%.6f,%.6f,%.6f", set == FDAS_SET_A ? 'A' : 'B',
+                double(t[EV_START]), double(t[EV_INPUT]), double(t[EV_FTC_K]), double(t[EV_FTC]),
+                double(t[EV_HSUM_K0]), double(t[EV_HSUM_K1]), double(t[EV_HSUM]), double(t[EV_XFER]));
   emit(ctx, FDAS_LOG_INFO, buf);
 }
 

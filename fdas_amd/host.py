@@ -264,8 +264,8 @@ class FDAS:
         self._lib.fdas_print_events(self._ctx, ab)
 
     def stage_times(self, ab) -> np.ndarray:
-        """[input, FT convolution, harmonic summing, read-back, total] in ms of the last run on ab."""
-        out = np.zeros(5, dtype=np.float32)
+        """[input, FT-conv kernel, HSUM kernel, selection, read-back, total] in ms of the last run on ab."""
+        out = np.zeros(6, dtype=np.float32)
         self._lib.fdas_get_stage_times(self._ctx, ab, N.fptr(out))
         return out
 

@@ -177,9 +177,10 @@ fdas_status fdas_retrieve_harmonic_plane(fdas_ctx* ctx, uint32_t k, float* plane
 void fdas_print_stats(fdas_ctx* ctx, int set, int reset);
 void fdas_print_events(fdas_ctx* ctx, int set);
 
-/* Durations in ms of the last launch on `set`: [0] input, [1] FT convolution,
- * [2] harmonic summing + selection, [3] candidate read-back, [4] total. */
-fdas_status fdas_get_stage_times(fdas_ctx* ctx, int set, float* ms5);
+/* Device durations in ms of the last run on `set` (HIP events on the set's
+ * stream): [0] input, [1] FT-convolution kernel, [2] harmonic-summing kernel,
+ * [3] candidate selection, [4] candidate read-back, [5] total. */
+fdas_status fdas_get_stage_times(fdas_ctx* ctx, int set, float* ms6);
 
 /* Device enumeration for the platform / device selectors of
  * initialise_accelerator (FDAS.h:45-47, FDAS.cpp:65-115).  fdas_device_info

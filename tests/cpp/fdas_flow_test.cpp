@@ -32,9 +32,10 @@ using fdas_amd::float2_t;
 
 namespace {
 
-// Minimal .npy reader: little-endian, C order, dtypes <f4, <c8, <u4.
+// Minimal .npy reader: little-endian, C order, dtypes <f4, <c8, <u4; `per` values of T per element.
 template <typename T>
-bool load_npy(const std::string& path, const char* want, std::vector<T>& out, std::vector<size_t>& shape) {
+bool load_npy(const std::string& path, const char* want, std::vector<T>& out, std::vector<size_t>& shape,
+              size_t per = 1) {
   std::ifstream f(path, std::ios::binary);
   if (!f) return false;
   char magic[6];
@@ -64,8 +65,8 @@ bool load_npy(const std::string& path, const char* want, std::vector<T>& out, st
     shape.push_back(std::stoul(tok));
     count *= shape.back();
   }
-  out.resize(count);
-  f.read(reinterpret_cast<char*>(out.data()), std::streamsize(count * sizeof(T)));
+  out.resize(count * per);
+  f.read(reinterpret_cast<char*>(out.data()), std::streamsize(count * per * sizeof(T)));
   return bool(f);
 }
 
@@ -188,6 +189,8 @@ void test_steps(const Vectors& v) {
   pipeline.configure(cfg);
   REQUIRE(pipeline.initialise_accelerator("", FDAS::choose_first_platform, FDAS::choose_accelerator_devices,
                                           uint32_t(v.input.size())));
+  REQUIRE(pipeline.get_templates_sz() == v.templates.size());
+  REQUIRE(pipeline.get_thresholds_sz() == v.thresholds.size());
   REQUIRE(pipeline.upload_templates(v.templates.data(), FDAS::B));
   REQUIRE(pipeline.upload_thresholds(v.thresholds.data(), FDAS::B));
   REQUIRE(pipeline.perform_input_tiling(v.input.data(), FDAS::B));
@@ -207,6 +210,7 @@ void drive_serially(const Vectors& v, FDAS::BufferMode mode, const std::string& 
   pipeline.configure(cfg);
   REQUIRE(pipeline.initialise_accelerator("", FDAS::choose_first_platform, FDAS::choose_accelerator_devices,
                                           uint32_t(v.input.size()), mode));
+  REQUIRE(pipeline.get_templates_sz() == v.templates.size());
   REQUIRE(pipeline.upload_templates(v.templates.data(), FDAS::A));
   std::vector<uint32_t> loc(pipeline.get_candidate_list_sz());
   std::vector<float> pwr(pipeline.get_candidate_list_sz());
@@ -228,9 +232,10 @@ void drive_pipelined(const Vectors& v, FDAS::BufferMode mode, const std::string&
   pipeline.configure(cfg);
   REQUIRE(pipeline.initialise_accelerator("", FDAS::choose_first_platform, FDAS::choose_accelerator_devices,
                                           uint32_t(v.input.size()), mode, true));
+  REQUIRE(pipeline.get_templates_sz() == v.templates.size());
   REQUIRE(pipeline.upload_templates(v.templates.data(), FDAS::A));
   REQUIRE(pipeline.upload_templates(v.templates.data(), FDAS::B));
-  std::vector<uint32_t> loc[2], pwr_dummy;
+  std::vector<uint32_t> loc[2];
   std::vector<float> pwr[2];
   for (int ab = 0; ab < 2; ++ab) {
     loc[ab].resize(pipeline.get_candidate_list_sz());
@@ -259,10 +264,10 @@ void drive_pipelined(const Vectors& v, FDAS::BufferMode mode, const std::string&
 bool load_vectors(const std::string& dir, const std::string& templates_file, Vectors& v) {
   std::vector<size_t> shape;
   std::vector<float> c;
-  if (!load_npy(dir + "/input.npy", "<c8", c, shape)) return false;
+  if (!load_npy(dir + "/input.npy", "<c8", c, shape, 2)) return false;
   v.input.resize(c.size() / 2);
   std::memcpy(v.input.data(), c.data(), c.size() * sizeof(float));
-  if (!load_npy(templates_file, "<c8", c, shape)) return false;
+  if (!load_npy(templates_file, "<c8", c, shape, 2)) return false;
   v.templates.resize(c.size() / 2);
   std::memcpy(v.templates.data(), c.data(), c.size() * sizeof(float));
   return load_npy(dir + "/thresholds.npy", "<f4", v.thresholds, shape) &&

@@ -288,7 +288,7 @@ def test_pipelined_launches_match_blocking(sync):
             assert f.wait(prev), f.last_error
             loc, pwr = expect[which[prev]]
             assert np.array_equal(locs[prev], loc) and np.array_equal(pwrs[prev], pwr)
-            assert f.stage_times(prev)[4] > 0
+            assert f.stage_times(prev)[5] > 0 and f.stage_times(prev)[1] > 0
     f.close()
 
 
