@@ -96,7 +96,7 @@ void report(const std::string& name, bool ok, const std::string& why = "") {
 
 // main_gtest.cpp:192-228, with the FP32-vs-FP64 threshold band.
 std::string validate_candidates(const Vectors& v, std::vector<uint32_t> loc, std::vector<float> pwr, int c) {
-  const uint32_t invalid = (uint32_t((c + 1) & 0x7f) << 22);
+  const uint32_t invalid = uint32_t((2 * c + 1) & 0x7f) << 22;  // encode(1, c+1, 0), gen_info.h:69
   size_t kept = 0;
   for (size_t i = 0; i < loc.size(); ++i)
     if (loc[i] != invalid) {
