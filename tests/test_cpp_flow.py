@@ -58,4 +58,4 @@ def test_reference_flows_in_cpp(tmp_path):
                         str(tmp_path)], capture_output=True, text=True, timeout=180)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr[-3000:]
-    assert r.stdout.count("  OK  ") == 10
+    assert r.stdout.count("  OK  ") == 11 and "FAIL" not in r.stdout
