@@ -27,6 +27,7 @@ using namespace fdas;
 namespace {
 
 constexpr uint32_t kDefaultSpill = 262144;
+constexpr uint32_t kMaxFtcJobs = 64;  // row groups per FT-convolution launch
 constexpr double kPi = 3.14159265358979323846264338327950288;
 
 // Events per launch: START, INPUT (spectrum in place), FTC_K (FT-convolution kernel start), FTC (its end),
@@ -67,6 +68,7 @@ struct DevSet {
   bool hsum_pending = false;
   bool ev_recorded[EV_COUNT] = {};
 
+  std::vector<uint32_t> row_taps;  // non-zero extent of each template row (sets per-group overlaps)
   bool templates = false;
   bool has_input = false;
   bool fop_valid = false;
@@ -208,31 +210,80 @@ uint32_t ftc_rows_per_item(uint32_t n_rows) {
   return (n_rows + chunks - 1) / chunks;
 }
 
+// Splits rows [rb, rb+nr) into contiguous groups, each convolved with its own
+// overlap: the longest non-zero template of the group minus one (rounded up to
+// 4), instead of the bank-wide max_taps - 1.  Overlap-save returns the same
+// linear convolution for any overlap >= L - 1, so results do not change; the
+// work does (payload S - O per transform).  Minimises
+//   sum over groups of n_tiles(O) * (rows + forward transforms per tile)
+// by dynamic programming over the row boundaries.
+std::vector<FtcJob> plan_ftc_jobs(const fdas_ctx* ctx, const DevSet& s, uint32_t rb, uint32_t nr) {
+  const uint32_t S = ctx->cfg.tile, N = ctx->geo.n_bins;
+  auto taps = [&](uint32_t r) { return s.row_taps.empty() ? ctx->cfg.max_taps : std::max(1u, s.row_taps[r]); };
+  auto tiles_for = [&](uint32_t o) { return (N + (S - o) - 1) / (S - o); };
+  std::vector<double> best(nr + 1, 1e300);
+  std::vector<uint32_t> from(nr + 1, 0);
+  best[0] = 0;
+  for (uint32_t j = 1; j <= nr; ++j) {
+    uint32_t lmax = 0;
+    for (uint32_t i = j; i-- > 0;) {
+      lmax = std::max(lmax, taps(rb + i));
+      const uint32_t o = overlap_for(lmax);
+      const uint32_t len = j - i;
+      const uint32_t rpi = ftc_rows_per_item(len);
+      const double cost = double(tiles_for(o)) * (double(len) + double((len + rpi - 1) / rpi));
+      if (best[i] + cost < best[j]) {
+        best[j] = best[i] + cost;
+        from[j] = i;
+      }
+    }
+  }
+  std::vector<std::pair<uint32_t, uint32_t>> segs;
+  for (uint32_t j = nr; j > 0; j = from[j]) segs.emplace_back(from[j], j);
+  std::reverse(segs.begin(), segs.end());
+  // merge the tail into at most kMaxFtcJobs groups (never needed for T <= 255 in practice)
+  while (segs.size() > kMaxFtcJobs) {
+    segs[segs.size() - 2].second = segs.back().second;
+    segs.pop_back();
+  }
+  std::vector<FtcJob> jobs;
+  uint32_t item = 0;
+  for (const auto& sg : segs) {
+    uint32_t lmax = 0;
+    for (uint32_t r = sg.first; r < sg.second; ++r) lmax = std::max(lmax, taps(rb + r));
+    FtcJob j{};
+    j.spectrum = s.input;
+    j.fop = s.fop;
+    j.hperm = s.hperm;
+    j.n_bins = N;
+    j.overlap = overlap_for(lmax);
+    j.payload = S - j.overlap;
+    j.n_tiles = tiles_for(j.overlap);
+    j.row_begin = rb + sg.first;
+    j.n_rows = sg.second - sg.first;
+    j.rows_per_item = ftc_rows_per_item(j.n_rows);
+    j.n_chunks = (j.n_rows + j.rows_per_item - 1) / j.rows_per_item;
+    j.item_begin = item;
+    item += j.n_tiles * j.n_chunks;
+    jobs.push_back(j);
+  }
+  return jobs;
+}
+
 fdas_status enqueue_ftc(fdas_ctx* ctx, DevSet& s, fdas_part part) {
   uint32_t rb, nr;
   part_rows(ctx, part, &rb, &nr);
   fdas_status st = wait_staging(ctx, s.staged_ftc, s.ftc_pending);
   if (st != FDAS_OK) return st;
-  FtcJob& j = *s.h_ftc;
-  j = FtcJob{};
-  j.spectrum = s.input;
-  j.fop = s.fop;
-  j.hperm = s.hperm;
-  j.n_bins = ctx->geo.n_bins;
-  j.overlap = ctx->geo.overlap;
-  j.payload = ctx->geo.payload;
-  j.n_tiles = ctx->geo.n_tiles;
-  j.row_begin = rb;
-  j.n_rows = nr;
-  j.rows_per_item = ftc_rows_per_item(nr);
-  j.n_chunks = (nr + j.rows_per_item - 1) / j.rows_per_item;
-  j.item_begin = 0;
-  const uint32_t n_items = j.n_tiles * j.n_chunks;
-  HIPCK(ctx, hipMemcpyAsync(s.d_ftc, s.h_ftc, sizeof(FtcJob), hipMemcpyHostToDevice, s.stream));
+  const std::vector<FtcJob> jobs = plan_ftc_jobs(ctx, s, rb, nr);
+  std::copy(jobs.begin(), jobs.end(), s.h_ftc);
+  const uint32_t n_items = jobs.back().item_begin + jobs.back().n_tiles * jobs.back().n_chunks;
+  HIPCK(ctx, hipMemcpyAsync(s.d_ftc, s.h_ftc, sizeof(FtcJob) * jobs.size(), hipMemcpyHostToDevice, s.stream));
   HIPCK(ctx, hipEventRecord(s.staged_ftc, s.stream));
   s.ftc_pending = true;
   if ((st = record(ctx, s, EV_FTC_K)) != FDAS_OK) return st;
-  HIPCK(ctx, launch_ftconv(int(ctx->cfg.tile), s.d_ftc, 1, n_items, ctx->tw1, ctx->tw2, n_items, s.stream));
+  HIPCK(ctx, launch_ftconv(int(ctx->cfg.tile), s.d_ftc, uint32_t(jobs.size()), n_items, ctx->tw1, ctx->tw2,
+                           n_items, s.stream));
   return FDAS_OK;
 }
 
@@ -506,10 +557,10 @@ extern "C" fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void
     if ((st = dev_alloc(ctx, &s.out_key, H * cap)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.out_pwr, H * cap)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.out_n, H)) != FDAS_OK) return bail(st);
-    if ((st = dev_alloc(ctx, &s.d_ftc, 1)) != FDAS_OK) return bail(st);
+    if ((st = dev_alloc(ctx, &s.d_ftc, kMaxFtcJobs)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.d_hsum, 1)) != FDAS_OK) return bail(st);
     if ((st = dev_alloc(ctx, &s.d_sel, H)) != FDAS_OK) return bail(st);
-    if ((st = host_alloc(ctx, &s.h_ftc, 1)) != FDAS_OK) return bail(st);
+    if ((st = host_alloc(ctx, &s.h_ftc, kMaxFtcJobs)) != FDAS_OK) return bail(st);
     if ((st = host_alloc(ctx, &s.h_hsum, 1)) != FDAS_OK) return bail(st);
     if ((st = host_alloc(ctx, &s.h_sel, H)) != FDAS_OK) return bail(st);
     if ((st = host_alloc(ctx, &s.h_out_key, H * cap)) != FDAS_OK) return bail(st);
@@ -597,6 +648,8 @@ static inline uint32_t p4_natural_index(uint32_t i, uint32_t S) {
 extern "C" fdas_status fdas_upload_templates(fdas_ctx* ctx, const float* spectra_c64, int set) {
   CHECK_SET(ctx, set);
   if (!spectra_c64) return fail(ctx, FDAS_EINVAL, "null template pointer");
+  // spectra do not reveal the tap extent exactly: every row uses the configured max_taps
+  ctx->set[set].row_taps.assign(ctx->cfg.n_templates, ctx->cfg.max_taps);
   const uint32_t S = ctx->cfg.tile, T = ctx->cfg.n_templates;
   const bool p4 = (ctx->cfg.flags & FDAS_FLAG_REFERENCE_FFT_ORDER) != 0;
   std::vector<std::complex<double>> spec(size_t(T) * S);
@@ -622,16 +675,23 @@ extern "C" fdas_status fdas_upload_template_taps(fdas_ctx* ctx, const float* tap
                     ctx->cfg.max_taps);
     }
   std::vector<std::complex<double>> spec(size_t(T) * S);
+  std::vector<uint32_t> extent(T, 1);
   for (uint32_t t = 0; t < T; ++t) {
     std::complex<double>* row = spec.data() + size_t(t) * S;
     const uint32_t m_end = std::min(n_taps, ctx->cfg.max_taps);
+    for (uint32_t m = 0; m < m_end; ++m) {
+      const float* v = taps_c64 + 2 * (size_t(t) * n_taps + m);
+      if (v[0] != 0.f || v[1] != 0.f) extent[t] = m + 1;
+    }
     for (uint32_t m = 0; m < m_end; ++m) {
       const float* v = taps_c64 + 2 * (size_t(t) * n_taps + m);
       row[m] = std::complex<double>(v[0], v[1]);
     }
     fft_inplace(row, S, -1);
   }
-  return upload_spectra(ctx, set, spec);
+  fdas_status st = upload_spectra(ctx, set, spec);
+  if (st == FDAS_OK) ctx->set[set].row_taps = extent;
+  return st;
 }
 
 extern "C" fdas_status fdas_upload_thresholds(fdas_ctx* ctx, const float* thresholds, int set) {

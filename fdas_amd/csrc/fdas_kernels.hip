@@ -223,28 +223,27 @@ __device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __res
   }
 }
 
+// split[k-1] = first of the thread's 4 bins whose sub-harmonic bin (f0+b)/k is
+// the upper one: k - (f0 mod k) (f0 is a multiple of 4, so the 4 bins span at
+// most two sub-harmonic bins for every k >= 2).  Per thread, fixed for all rows.
 template <int H>
 __device__ __forceinline__ void hsum_row(HsumStage& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
-                                         uint32_t f0, uint32_t n_bins, const float (&thr)[H], uint32_t (&wn)[H],
-                                         uint32_t wave) {
-  const uint32_t fa = min(f0, n_bins - 1);
-  const uint32_t lane = lane_id();
+                                         uint32_t f0, uint32_t n_valid, const uint32_t (&split)[H],
+                                         const float (&thr)[H], uint32_t (&wn)[H], uint32_t wave) {
   float hp[kHsumBinsPerThread];
 #pragma unroll
   for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] = L.m[b];
 #pragma unroll
   for (int k = 1; k <= H; ++k) {
     if (k > 1) {
-      const uint32_t qa = fa / uint32_t(k);
 #pragma unroll
-      for (int b = 0; b < kHsumBinsPerThread; ++b)
-        hp[b] += ((f0 + uint32_t(b)) / uint32_t(k) == qa) ? L.lo[k - 1] : L.hi[k - 1];
+      for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] += (uint32_t(b) >= split[k - 1]) ? L.hi[k - 1] : L.lo[k - 1];
     }
     bool hit[kHsumBinsPerThread];
     bool any = false;
 #pragma unroll
     for (int b = 0; b < kHsumBinsPerThread; ++b) {
-      hit[b] = f0 + uint32_t(b) < n_bins && hp[b] > thr[k - 1];
+      hit[b] = uint32_t(b) < n_valid && hp[b] > thr[k - 1];
       any |= hit[b];
     }
     if (__ballot(any) == 0) continue;
@@ -265,7 +264,6 @@ __device__ __forceinline__ void hsum_row(HsumStage& st, const HsumJob& job, cons
       wn[k - 1] += cnt;
     }
   }
-  (void)lane;
 }
 
 // HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] in FP32, in k order (hsum.py:331);
@@ -289,21 +287,23 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   const uint32_t r_begin = job.row_begin + chunk * job.rows_per_block;
   const uint32_t r_last = min(r_begin + job.rows_per_block, job.row_begin + job.n_rows) - 1;
   float thr[H];
-  uint32_t wn[H];
+  uint32_t wn[H], split[H];
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     thr[k] = job.thr[k];
     wn[k] = 0;
+    split[k] = uint32_t(k + 1) - f0 % uint32_t(k + 1);
   }
+  const uint32_t n_valid = f0 < n_bins ? min(n_bins - f0, uint32_t(kHsumBinsPerThread)) : 0u;
 
   RowLoads<H> a, b;
   hsum_load_row<H, VEC>(a, fop, n_bins, c, r_begin, f0);
   for (uint32_t r = r_begin;; r += 2) {
     hsum_load_row<H, VEC>(b, fop, n_bins, c, min(r + 1, r_last), f0);
-    hsum_row<H>(st, job, a, r, f0, n_bins, thr, wn, wave);
+    hsum_row<H>(st, job, a, r, f0, n_valid, split, thr, wn, wave);
     if (r + 1 > r_last) break;
     hsum_load_row<H, VEC>(a, fop, n_bins, c, min(r + 2, r_last), f0);
-    hsum_row<H>(st, job, b, r + 1, f0, n_bins, thr, wn, wave);
+    hsum_row<H>(st, job, b, r + 1, f0, n_valid, split, thr, wn, wave);
     if (r + 2 > r_last) break;
   }
   // Flush what is left: one global atomic per plane for the whole workgroup.

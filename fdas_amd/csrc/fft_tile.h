@@ -190,6 +190,15 @@ __device__ __forceinline__ void dftD(float2 (&v)[N]) {
   }
 }
 
+// LDS row padding of the two exchanges (float2 units), chosen by measurement
+// of LDS bank conflicts on gfx950.
+#ifndef FDAS_RS1_PAD
+#define FDAS_RS1_PAD 1
+#endif
+#ifndef FDAS_RS2_PAD
+#define FDAS_RS2_PAD 16
+#endif
+
 // Compile-time layout of one S-point tile transform.
 template <int S>
 struct TileFFT {
@@ -197,8 +206,8 @@ struct TileFFT {
   static constexpr int NT = S / 16;     // threads per transform
   static constexpr int D = NT / 16;     // radix of the last pass
   static constexpr int PAIRS = 16 / D;  // (ka, kc) pairs per thread in pass 3
-  static constexpr int RS1 = NT + 2;    // row stride of the [ka][b] exchange
-  static constexpr int RS2 = 256 + 16;  // row stride of the [d][q] exchange
+  static constexpr int RS1 = NT + FDAS_RS1_PAD;    // row stride of the [ka][b] exchange
+  static constexpr int RS2 = 256 + FDAS_RS2_PAD;   // row stride of the [d][q] exchange
   static constexpr int L1_ELEMS = 16 * RS1;
   static constexpr int L2_ELEMS = D * RS2;
   static constexpr int LDS_ELEMS = L1_ELEMS > L2_ELEMS ? L1_ELEMS : L2_ELEMS;
