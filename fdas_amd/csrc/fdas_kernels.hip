@@ -71,12 +71,22 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
     const float2* __restrict__ hbase = static_cast<const float2*>(job->hperm);
     const uint32_t out0 = tile * payload;  // FOP bin of tile sample s = overlap
 
+    // The template spectrum of the next row is loaded before this row's FOP
+    // stores: loads and stores share one in-order completion counter, so a
+    // load issued after the stores would wait for them too.
+    float2 hn[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) hn[s] = gld2(hbase + size_t(r_begin) * S + tid, size_t(s * NT));
     for (uint32_t r = r_begin; r < r_end; ++r) {
-      const float2* __restrict__ h = hbase + size_t(r) * S + tid;
       float2 v[16];
 #pragma unroll
-      for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], gld2(h, size_t(s * NT)));
+      for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], hn[s]);
       fft.inverse(v, la, lb, tid);
+      {
+        const float2* __restrict__ h = hbase + size_t(min(r + 1, r_end - 1)) * S + tid;  // unconditional: no branch
+#pragma unroll
+        for (int s = 0; s < 16; ++s) hn[s] = gld2(h, size_t(s * NT));
+      }
       float* __restrict__ out = job->fop + size_t(r) * n_bins;
 #pragma unroll
       for (int a = 0; a < 16; ++a) {
