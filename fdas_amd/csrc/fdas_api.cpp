@@ -21,6 +21,7 @@
 #include "fdas_host.h"
 #include "fdas_jobs.h"
 #include "fdas_kernels.h"
+#include "fdas_plan.h"
 
 using namespace fdas;
 
@@ -28,7 +29,6 @@ namespace {
 
 constexpr uint32_t kDefaultSpill = 262144;
 constexpr uint32_t kMaxFtcJobs = 64;  // row groups per FT-convolution launch
-constexpr double kPi = 3.14159265358979323846264338327950288;
 
 // Events per launch: START, INPUT (spectrum in place), FTC_K (FT-convolution kernel start), FTC (its end),
 // HSUM_K0 / HSUM_K1 (harmonic-summing kernel start / end), HSUM (selection end), XFER (lists read back).
@@ -202,70 +202,16 @@ fdas_status record(fdas_ctx* ctx, DevSet& s, Ev e) {
   return FDAS_OK;
 }
 
-uint32_t ftc_rows_per_item(uint32_t n_rows) {
-  // Enough rows per work item to amortise the forward transform, small enough
-  // for several items per tile (load balance over 256 CUs).
-  const uint32_t target = 16;
-  const uint32_t chunks = std::max<uint32_t>(1, (n_rows + target - 1) / target);
-  return (n_rows + chunks - 1) / chunks;
-}
-
-// Splits rows [rb, rb+nr) into contiguous groups, each convolved with its own
-// overlap: the longest non-zero template of the group minus one (rounded up to
-// 4), instead of the bank-wide max_taps - 1.  Overlap-save returns the same
-// linear convolution for any overlap >= L - 1, so results do not change; the
-// work does (payload S - O per transform).  Minimises
-//   sum over groups of n_tiles(O) * (rows + forward transforms per tile)
-// by dynamic programming over the row boundaries.
 std::vector<FtcJob> plan_ftc_jobs(const fdas_ctx* ctx, const DevSet& s, uint32_t rb, uint32_t nr) {
-  const uint32_t S = ctx->cfg.tile, N = ctx->geo.n_bins;
-  auto taps = [&](uint32_t r) { return s.row_taps.empty() ? ctx->cfg.max_taps : std::max(1u, s.row_taps[r]); };
-  auto tiles_for = [&](uint32_t o) { return (N + (S - o) - 1) / (S - o); };
-  std::vector<double> best(nr + 1, 1e300);
-  std::vector<uint32_t> from(nr + 1, 0);
-  best[0] = 0;
-  for (uint32_t j = 1; j <= nr; ++j) {
-    uint32_t lmax = 0;
-    for (uint32_t i = j; i-- > 0;) {
-      lmax = std::max(lmax, taps(rb + i));
-      const uint32_t o = overlap_for(lmax);
-      const uint32_t len = j - i;
-      const uint32_t rpi = ftc_rows_per_item(len);
-      const double cost = double(tiles_for(o)) * (double(len) + double((len + rpi - 1) / rpi));
-      if (best[i] + cost < best[j]) {
-        best[j] = best[i] + cost;
-        from[j] = i;
-      }
-    }
-  }
-  std::vector<std::pair<uint32_t, uint32_t>> segs;
-  for (uint32_t j = nr; j > 0; j = from[j]) segs.emplace_back(from[j], j);
-  std::reverse(segs.begin(), segs.end());
-  // merge the tail into at most kMaxFtcJobs groups (never needed for T <= 255 in practice)
-  while (segs.size() > kMaxFtcJobs) {
-    segs[segs.size() - 2].second = segs.back().second;
-    segs.pop_back();
-  }
-  std::vector<FtcJob> jobs;
+  std::vector<FtcJob> jobs = plan_ftc_rows(ctx->cfg.tile, ctx->geo.n_bins, ctx->cfg.max_taps, s.row_taps, rb, nr,
+                                           kMaxFtcJobs);
   uint32_t item = 0;
-  for (const auto& sg : segs) {
-    uint32_t lmax = 0;
-    for (uint32_t r = sg.first; r < sg.second; ++r) lmax = std::max(lmax, taps(rb + r));
-    FtcJob j{};
+  for (FtcJob& j : jobs) {
     j.spectrum = s.input;
     j.fop = s.fop;
     j.hperm = s.hperm;
-    j.n_bins = N;
-    j.overlap = overlap_for(lmax);
-    j.payload = S - j.overlap;
-    j.n_tiles = tiles_for(j.overlap);
-    j.row_begin = rb + sg.first;
-    j.n_rows = sg.second - sg.first;
-    j.rows_per_item = ftc_rows_per_item(j.n_rows);
-    j.n_chunks = (j.n_rows + j.rows_per_item - 1) / j.rows_per_item;
     j.item_begin = item;
     item += j.n_tiles * j.n_chunks;
-    jobs.push_back(j);
   }
   return jobs;
 }
@@ -303,8 +249,7 @@ fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* 
   j.n_harmonics = H;
   j.spill_cap = ctx->spill;
   j.block_begin = 0;
-  j.n_row_chunks = (nr + kHsumRowsPerBlock - 1) / kHsumRowsPerBlock;
-  j.rows_per_block = (nr + j.n_row_chunks - 1) / j.n_row_chunks;
+  hsum_row_chunks(nr, &j.rows_per_block, &j.n_row_chunks);
   j.spill_key = s.spill_key;
   j.spill_pwr = s.spill_pwr;
   j.counts = s.counts;
@@ -522,19 +467,10 @@ extern "C" fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void
   if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess)
     info(ctx, "Device[%d]: %s (%s, %d CUs)", cfg->device, prop.name, prop.gcnArchName, prop.multiProcessorCount);
 
-  // Twiddle tables in FP64, rounded once: w1[k][b] = W_S^(b k), w2[k][d] = W_NT^(d k).
-  const uint32_t S = cfg->tile, NT = S / 16, D = NT / 16;
-  std::vector<float2> tw1(16 * NT), tw2(16 * D);
-  for (uint32_t k = 0; k < 16; ++k) {
-    for (uint32_t b = 0; b < NT; ++b) {
-      const double a = -2.0 * kPi * double((uint64_t(b) * k) % S) / double(S);
-      tw1[k * NT + b] = make_float2(float(std::cos(a)), float(std::sin(a)));
-    }
-    for (uint32_t d = 0; d < D; ++d) {
-      const double a = -2.0 * kPi * double((uint64_t(d) * k) % NT) / double(NT);
-      tw2[k * D + d] = make_float2(float(std::cos(a)), float(std::sin(a)));
-    }
-  }
+  // Twiddle tables in FP64, rounded once.
+  const uint32_t S = cfg->tile;
+  std::vector<float2> tw1, tw2;
+  make_twiddles(S, tw1, tw2);
   if ((st = dev_alloc(ctx, &ctx->tw1, tw1.size())) != FDAS_OK) return bail(st);
   if ((st = dev_alloc(ctx, &ctx->tw2, tw2.size())) != FDAS_OK) return bail(st);
   if (hipMemcpy(ctx->tw1, tw1.data(), tw1.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
@@ -617,17 +553,8 @@ extern "C" uint64_t fdas_get_size(const fdas_ctx* ctx, fdas_size_kind which) {
 }
 
 static fdas_status upload_spectra(fdas_ctx* ctx, int set, const std::vector<std::complex<double>>& spec) {
-  // spec: T x S, natural order.  Stored as [t][slot][tid] (kernel register
-  // order after the forward tile transform), scaled by 1/S, which is exact.
-  const uint32_t S = ctx->cfg.tile, NT = S / 16, T = ctx->cfg.n_templates;
-  std::vector<float2> perm(size_t(T) * S);
-  const double scale = 1.0 / double(S);
-  for (uint32_t t = 0; t < T; ++t)
-    for (uint32_t slot = 0; slot < 16; ++slot)
-      for (uint32_t tid = 0; tid < NT; ++tid) {
-        const std::complex<double> v = spec[size_t(t) * S + tile_slot_bin(S, tid, slot)] * scale;
-        perm[size_t(t) * S + size_t(slot) * NT + tid] = make_float2(float(v.real()), float(v.imag()));
-      }
+  // Stored as [t][slot][tid] (kernel register order after the forward tile transform), scaled by 1/S.
+  const std::vector<float2> perm = kernel_template_layout(spec, ctx->cfg.n_templates, ctx->cfg.tile);
   DevSet& s = ctx->set[set];
   HIPCK(ctx, hipSetDevice(ctx->cfg.device));
   HIPCK(ctx, hipStreamSynchronize(s.stream));
@@ -674,21 +601,8 @@ extern "C" fdas_status fdas_upload_template_taps(fdas_ctx* ctx, const float* tap
         return fail(ctx, FDAS_EINVAL, "template %u has a non-zero tap %u beyond max_taps %u", t, m,
                     ctx->cfg.max_taps);
     }
-  std::vector<std::complex<double>> spec(size_t(T) * S);
-  std::vector<uint32_t> extent(T, 1);
-  for (uint32_t t = 0; t < T; ++t) {
-    std::complex<double>* row = spec.data() + size_t(t) * S;
-    const uint32_t m_end = std::min(n_taps, ctx->cfg.max_taps);
-    for (uint32_t m = 0; m < m_end; ++m) {
-      const float* v = taps_c64 + 2 * (size_t(t) * n_taps + m);
-      if (v[0] != 0.f || v[1] != 0.f) extent[t] = m + 1;
-    }
-    for (uint32_t m = 0; m < m_end; ++m) {
-      const float* v = taps_c64 + 2 * (size_t(t) * n_taps + m);
-      row[m] = std::complex<double>(v[0], v[1]);
-    }
-    fft_inplace(row, S, -1);
-  }
+  std::vector<uint32_t> extent;
+  const std::vector<std::complex<double>> spec = taps_to_spectra(taps_c64, T, n_taps, ctx->cfg.max_taps, S, extent);
   fdas_status st = upload_spectra(ctx, set, spec);
   if (st == FDAS_OK) ctx->set[set].row_taps = extent;
   return st;
