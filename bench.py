@@ -1,20 +1,31 @@
 #!/usr/bin/env python3
 """FDAS hot-path benchmark on one MI355X.
 
-Metric: DM-trials/sec at N = 2^21 bins, 85 templates, 8 harmonics (4,096
-candidates per harmonic), the single-trial configuration of the reference's
-search.  One step = one DM trial through the whole device path: fused FT
-convolution -> harmonic summing + thresholding + compaction -> per-plane
-selection -> candidate read-back.  Inputs (spectrum, template spectra) are
-resident in HBM before the timed region; two buffer sets A/B keep two trials
-in flight, as the reference's pipelined driver does (main_gtest.cpp:388-437).
+Metric: DM-trials/sec at N = 2^21 bins, 85 templates x 421 taps, 8 harmonics
+(4,096 candidates per harmonic).  A trial is one dedispersed spectrum through
+the whole device path: fused FT convolution -> harmonic summing + thresholding
++ compaction -> per-plane selection -> candidate lists in host memory.
 
-Also reported, in the same JSON line:
-  * plain GPU form: one trial per launch, host spectrum uploaded, candidates
-    read back and the stream synchronised per trial (p50 / p99 latency);
-  * roofline of the dominant kernel (HIP-event durations on its own stream);
-  * CPU baseline: the oracle's restatement of the reference path timed on a
-    bounded sample on this host.
+The timed form (``value``) is the batched engine on the batched-uniform
+workload: 64 trials of N = 2^21 (seeds 1000..1063, 0-2 drifting tones each),
+spectra resident in HBM before the timed region, one step = one batch of 64
+trials.  Two plans alternate so that one batch's tail overlaps the next
+batch's head; inside a plan, groups of trials pipeline FT convolution on one
+stream against harmonic summing on another.
+
+Also reported in the same JSON line:
+  * single_trial_form: the drop-in class FDAS path, two trials in flight (A/B);
+  * plain_gpu_form: one trial per launch, host spectrum uploaded, lists read
+    back and the stream synchronised per trial (p50 / p99 latency);
+  * graph_form: the same batches replayed as HIP graphs;
+  * uneven_form: 48 trials of N in {2^19..2^22}, banks of 85 / 169 templates,
+    8 or 16 harmonics, in one plan;
+  * served_form: open-loop Poisson request streams through the serving loop,
+    p50 / p99 latency and throughput beside the plain one-request-per-launch
+    form on the same arrivals;
+  * roofline of the dominant kernel (HIP-event spans on its own stream over
+    the timed region) and the CPU baseline (the oracle's C restatement timed
+    on this host).
 """
 
 from __future__ import annotations
@@ -23,6 +34,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -32,20 +44,25 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+N_METRIC = 1 << 21
+T_METRIC = 85
+H_METRIC = 8
+CAP = 4096
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def ftc_bytes(n_bins, n_templates, tile):
-    """Algorithmic HBM bytes of one FT-convolution launch: spectrum read + FOP write + template spectra."""
-    return 8 * n_bins + 4 * n_templates * n_bins + 8 * n_templates * tile
+def ftc_bytes(n_bins, n_templates, tile, trials=1):
+    """Algorithmic HBM bytes of one FT-convolution launch over `trials` trials:
+    spectrum read + FOP write per trial, template spectra once."""
+    return trials * (8 * n_bins + 4 * n_templates * n_bins) + 8 * n_templates * tile
 
 
-def hsum_bytes(n_bins, n_templates, n_cands):
-    """Algorithmic HBM bytes of one harmonic-summing launch: FOP read once + candidates out."""
-    return 4 * n_templates * n_bins + 8 * n_cands
+def hsum_bytes(n_bins, n_templates, n_cands, trials=1):
+    """Algorithmic HBM bytes of one harmonic-summing launch: FOP read once + candidates out, per trial."""
+    return trials * (4 * n_templates * n_bins) + 8 * n_cands
 
 
 def fft_flops(n_bins, n_templates, tile, overlap):
@@ -54,6 +71,18 @@ def fft_flops(n_bins, n_templates, tile, overlap):
     n_tiles = -(-n_bins // payload)
     per = 5.0 * tile * np.log2(tile)
     return n_tiles * per + n_tiles * n_templates * (per + 6 * tile + 3 * payload)
+
+
+def device_thresholds(x, n_harmonics, sigma=6.0):
+    """T_k = (k + sigma sqrt(k)) * mean|X|^2 (the noise_thresholds rule), mean taken on the device in FP64."""
+    import torch
+    p = float((x.real.double() ** 2 + x.imag.double() ** 2).mean().item())
+    k = np.arange(1, n_harmonics + 1, dtype=np.float64)
+    return ((k + sigma * np.sqrt(k)) * p).astype(np.float32)
+
+
+def pct(a, q):
+    return round(float(np.percentile(np.asarray(a), q)), 3) if len(a) else None
 
 
 def cpu_baseline(x, bank, thr, cap, budget_s=10.0):
@@ -80,40 +109,13 @@ def cpu_baseline(x, bank, thr, cap, budget_s=10.0):
             "ms_per_trial": round(med * 1000.0, 2)}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40, help="timed DM trials")
-    ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--in-flight", type=int, default=2, choices=(1, 2),
-                    help="trials in flight in the timed region (buffer sets A/B)")
-    ap.add_argument("--isolated-steps", type=int, default=10,
-                    help="extra one-at-a-time trials for uncontended kernel durations (0: skip)")
-    ap.add_argument("--plain-trials", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tile", type=int, default=2048)
-    args = ap.parse_args()
+# --------------------------------------------------------------------------- single-trial drop-in path
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the FDAS hot path shards DM trials across GPUs with no "
-              "collective, so a multi-GPU figure is N independent copies of this run (see DESIGN.md)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import fdas_amd
-    from fdas_amd import synth
+def single_trial_forms(args, x, bank, thr, forms):
     from fdas_amd.host import FDAS
-
-    w = synth.full_workload()
-    log(f"workload: {w.name}")
-    x = w.spectrum()
-    bank = fdas_amd.make_drift_bank(synth.drift_grid(w.n_templates), w.max_taps)
-    thr = synth.noise_thresholds(x, w.n_harmonics)
-    cap = w.cands_per_harmonic
-
     f = FDAS(log=lambda lvl, msg: (log(msg) if lvl >= 1 else None))
-    assert f.initialise_accelerator(w.n_bins, tile=args.tile, n_templates=w.n_templates, max_taps=w.max_taps,
-                                    n_harmonics=w.n_harmonics, cands_per_harmonic=cap), f.last_error
+    assert f.initialise_accelerator(N_METRIC, tile=2048, n_templates=T_METRIC, max_taps=421,
+                                    n_harmonics=H_METRIC, cands_per_harmonic=CAP), f.last_error
     for ab in (FDAS.A, FDAS.B):
         assert f.upload_template_taps(bank, ab), f.last_error
         assert f.perform_input_tiling(x, ab), f.last_error
@@ -123,7 +125,6 @@ def main():
     part = FDAS.AllAccelerations
 
     def run(n_steps, depth, record=None):
-        """n_steps trials, `depth` in flight on sets A/B; per-trial stage times appended to record."""
         in_flight = []
         for i in range(n_steps):
             ab = i % 2
@@ -139,36 +140,21 @@ def main():
             if record is not None:
                 record.append(f.stage_times(prev))
 
-    isolated = None
-    if args.isolated_steps > 0:
-        run(2, 1)
+    out = {}
+    if "single" in forms:
+        run(4, 2)
         rec = []
         t0 = time.perf_counter()
-        run(args.isolated_steps, 1, rec)
-        ms = (time.perf_counter() - t0) * 1000.0 / args.isolated_steps
+        run(30, 2, rec)
+        ms = (time.perf_counter() - t0) * 1000.0 / 30
         st = np.array(rec)
-        isolated = {"ms_per_trial": round(ms, 4), "trials_per_s": round(1000.0 / ms, 2),
-                    "ftconv_ms": round(float(st[:, 1].mean()), 4), "hsum_ms": round(float(st[:, 2].mean()), 4),
-                    "select_ms": round(float(st[:, 3].mean()), 4)}
-        log(f"isolated (1 in flight): {isolated}")
-
-    log(f"warmup {args.warmup} trials, {args.in_flight} in flight")
-    run(args.warmup, args.in_flight)
-    rec = []
-    log(f"timing {args.steps} trials")
-    t0 = time.perf_counter()
-    run(args.steps, args.in_flight, rec)
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    value = 1000.0 / ms_per_step
-    st = np.array(rec)
-    ftc_ms, hsum_ms, sel_ms = (float(st[:, i].mean()) for i in (1, 2, 3))
-    log(f"timed: {ms_per_step:.3f} ms/trial ({value:.1f} trials/s); kernel means ftconv {ftc_ms:.3f} ms, "
-        f"hsum {hsum_ms:.3f} ms, select {sel_ms:.3f} ms")
-
-    # Plain GPU form: one trial per launch, host upload, read-back and synchronisation each time.
-    plain = None
-    if args.plain_trials > 0:
+        out["single_trial_form"] = {
+            "value": round(1000.0 / ms, 2), "unit": "DM-trials/s", "ms_per_trial": round(ms, 4),
+            "ftconv_ms": round(float(st[:, 1].mean()), 4), "hsum_ms": round(float(st[:, 2].mean()), 4),
+            "select_ms": round(float(st[:, 3].mean()), 4),
+            "form": "class FDAS drop-in path, launch_resident on buffer sets A/B, two trials in flight"}
+        log(f"single-trial: {out['single_trial_form']}")
+    if "plain" in forms:
         lat = []
         for i in range(args.plain_trials + 2):
             t = time.perf_counter()
@@ -177,57 +163,363 @@ def main():
             if i >= 2:
                 lat.append((time.perf_counter() - t) * 1000.0)
         lat = np.array(lat)
-        plain = {"value": round(float(1000.0 / lat.mean()), 2), "unit": "DM-trials/s",
-                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                 "form": "host spectrum upload + FT conv + HSUM + selection + read-back + sync, one trial per launch"}
-        log(f"plain: {plain}")
+        out["plain_gpu_form"] = {
+            "value": round(float(1000.0 / lat.mean()), 2), "unit": "DM-trials/s",
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "form": "host spectrum upload + FT conv + HSUM + selection + read-back + sync, one trial per launch"}
+        log(f"plain: {out['plain_gpu_form']}")
+    f.close()
+    return out
 
-    _, _, _, _, det = f.retrieve_candidates_wide(FDAS.A)
-    n_cands = int(np.minimum(det, cap).sum())
-    kernels = {
-        "ftconv_kernel<2048>": {"ms": ftc_ms, "bytes": ftc_bytes(w.n_bins, w.n_templates, args.tile)},
-        "hsum_kernel<8,true>": {"ms": hsum_ms, "bytes": hsum_bytes(w.n_bins, w.n_templates, n_cands)},
-    }
+
+# --------------------------------------------------------------------------- batched engine
+
+def make_engine(args, **kw):
+    from fdas_amd.engine import Engine
+    return Engine(tile=args.tile, rows_per_item=args.rows_per_item, hsum_rows=args.hsum_rows,
+                  cands_per_harmonic=CAP, group_trials=args.group_trials,
+                  ftc_workgroups_per_cu=args.ftc_wg_per_cu,
+                  log=lambda lvl, msg: (log(msg) if (lvl >= 1 or args.verbose) else None), **kw)
+
+
+def run_batches(plans, n, on_done=None):
+    """n batches, alternating over the plans, at most len(plans) in flight."""
+    in_flight = []
+    for i in range(n):
+        p = plans[i % len(plans)]
+        if len(in_flight) == len(plans):
+            q = in_flight.pop(0)
+            q.wait(results=False)
+            if on_done:
+                on_done(q)
+        p.launch()
+        in_flight.append(p)
+    for q in in_flight:
+        q.wait(results=False)
+        if on_done:
+            on_done(q)
+
+
+def batched_form(args, eng, bank85):
+    import torch
+    from fdas_amd import synth
+    from fdas_amd.engine import Trial
+    log(f"batched: generating {args.batch} trials of N=2^21 on the device")
+    trials = []
+    for i in range(args.batch):
+        w = synth.random_trial(1000 + i, N_METRIC)
+        x = synth.device_spectrum(w)
+        trials.append(Trial(x, bank85, device_thresholds(x, H_METRIC), 2, tag=1000 + i))
+    torch.cuda.synchronize()
+    tile = eng.prepare(N_METRIC, bank85, 2)
+    out = {"tile": tile}
+    spans = {"ftc": [], "hsum": [], "groups": 0}
+
+    def collect(p):
+        info = p.info()
+        spans["ftc"].append(info["ftc_ms"] / info["n_groups"])
+        spans["hsum"].append(info["hsum_ms"] / info["n_groups"])
+
+    plans = [eng.plan(trials) for _ in range(2)]
+    out["groups_per_batch"] = plans[0].info()["n_groups"]
+    out["state_gb_per_plan"] = round(plans[0].info()["region_bytes"] / 1e9, 2)
+    run_batches(plans, args.warmup)
+    log(f"batched: timing {args.steps} batches of {args.batch}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_batches(plans, args.steps, collect)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    out["ms_per_step"] = ms_per_step
+    out["value"] = args.batch * 1000.0 / ms_per_step
+    out["ftc_span_ms"] = float(np.mean(spans["ftc"]))
+    out["hsum_span_ms"] = float(np.mean(spans["hsum"]))
+    # Results of one batch, for the record: every trial completed, no spill overflow.
+    plans[0].launch()
+    res = plans[0].wait()
+    out["statuses_ok"] = all(r.status == 0 for r in res)
+    out["candidates_per_trial_mean"] = float(np.mean([int(np.sum(r.n_kept)) for r in res]))
+    for p in plans:
+        p.close()
+    log(f"batched eager: {out['value']:.1f} trials/s, {ms_per_step:.2f} ms per batch, tile {tile}, "
+        f"spans ftc {out['ftc_span_ms']:.3f} ms hsum {out['hsum_span_ms']:.3f} ms per group")
+    if not args.no_graph:
+        gplans = [eng.plan(trials, graph=True) for _ in range(2)]
+        run_batches(gplans, args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_batches(gplans, args.steps)
+        torch.cuda.synchronize()
+        gms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        out["graph"] = {"value": round(args.batch * 1000.0 / gms, 2), "ms_per_step": round(gms, 4)}
+        log(f"batched graph: {out['graph']}")
+        for p in gplans:
+            p.close()
+    del trials
+    return out
+
+
+def uneven_form(args, eng, banks):
+    import torch
+    from fdas_amd import synth
+    from fdas_amd.engine import Trial
+    trials, bins = [], 0
+    sizes = [1 << 19, 1 << 20, 1 << 21, 1 << 22]
+    for i in range(48):
+        r = np.random.default_rng(2000 + i)
+        n = int(r.choice(sizes))
+        T = int(r.choice([85, 169]))
+        H = 16 if i % 4 == 0 else 8
+        w = synth.random_trial(2000 + i, n, z_max=84.0 if T == 85 else 168.0)
+        x = synth.device_spectrum(w)
+        trials.append(Trial(x, banks[T], device_thresholds(x, H), 2, tag=2000 + i))
+        bins += n * T
+    torch.cuda.synchronize()
+    for t in trials:
+        eng.prepare(int(t.spectrum.numel()), t.bank, 2)
+    tiles = {}
+    plan = eng.plan(trials, graph=not args.no_graph)
+    run_batches([plan], 1)
+    n = max(2, args.steps // 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_batches([plan], n)
+    ms = (time.perf_counter() - t0) * 1000.0 / n
+    res = plan.run()
+    for t, r in zip(trials, res):
+        key = f"N={int(t.spectrum.numel())},T={85 if t.bank == banks[85] else 169}"
+        tiles[key] = r.tile
+    plan.close()
+    out = {"value": round(48 * 1000.0 / ms, 2), "unit": "DM-trials/s", "ms_per_batch": round(ms, 3),
+           "fop_gsamples_per_s": round(bins / (ms * 1e-3) / 1e9, 1), "tiles": tiles,
+           "statuses_ok": all(r.status == 0 for r in res), "graph": not args.no_graph,
+           "workload": "48 trials, N in {2^19,2^20,2^21,2^22}, banks 85 (z in [-84,84]) / 169 (z in [-168,168]) "
+                       "x 421 taps, H = 16 for every 4th trial else 8"}
+    log(f"uneven: {out}")
+    return out
+
+
+# --------------------------------------------------------------------------- serving
+
+def served_form(args, eng, banks, bank_taps):
+    import torch
+    from fdas_amd import synth
+    from fdas_amd.engine import Trial
+    from fdas_amd.host import FDAS
+    sizes, weights = [1 << 20, 1 << 21, 1 << 22], [0.3, 0.5, 0.2]
+    pool = {}
+    for n in sizes:
+        for j in range(3):
+            for T in (85, 169):
+                w = synth.random_trial(3000 + 10 * j + (n.bit_length()), n, z_max=84.0 if T == 85 else 168.0)
+                x = synth.device_spectrum(w)
+                pool.setdefault((n, T), []).append((x, device_thresholds(x, H_METRIC)))
+    torch.cuda.synchronize()
+    for (n, T) in pool:
+        eng.prepare(n, banks[T], 2)
+
+    # plain one-request-per-launch server: one class FDAS context per shape
+    plain_ctx = {}
+    for (n, T) in pool:
+        f = FDAS(log=lambda lvl, msg: (log(msg) if lvl >= 1 else None))
+        assert f.initialise_accelerator(n, n_templates=T, max_taps=421, tile=2048, n_harmonics=H_METRIC,
+                                        cands_per_harmonic=CAP), f.last_error
+        assert f.upload_template_taps(bank_taps[T], FDAS.A), f.last_error
+        plain_ctx[(n, T)] = f
+
+    def plain_one(key, x, thr):
+        f = plain_ctx[key]
+        assert f.set_input_device(x.data_ptr(), FDAS.A), f.last_error
+        assert f.perform_ft_convolution(FDAS.AllAccelerations, FDAS.A), f.last_error
+        assert f.perform_harmonic_summing(thr, FDAS.AllAccelerations, FDAS.A), f.last_error
+        f.retrieve_candidates_wide(FDAS.A)
+
+    # warm every path once
+    for key, lst in pool.items():
+        plain_one(key, *lst[0])
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    out = {"rates": {}, "workload": "Poisson arrivals; N in {2^20,2^21,2^22} (0.3/0.5/0.2), banks 85/169 x 421 taps, "
+                                    "H = 8, 4096 candidates/plane, spectra resident in HBM",
+           "hbm_budget_gb": 200}
+    for rate in rates:
+        rng = np.random.default_rng(int(rate) + 17)
+        n_req = max(40, int(rate * args.serve_seconds))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        picks = []
+        for i in range(n_req):
+            n = int(rng.choice(sizes, p=weights))
+            T = int(rng.choice([85, 169]))
+            x, thr = pool[(n, T)][int(rng.integers(0, 3))]
+            picks.append(((n, T), x, thr))
+        # served form
+        results = []
+        done_evt = threading.Event()
+
+        def poller():
+            while len(results) < n_req:
+                results.extend(eng.poll(max_results=64, timeout_ms=200))
+            done_evt.set()
+
+        th = threading.Thread(target=poller, daemon=True)
+        th.start()
+        t_start = time.perf_counter()
+        for i, (key, x, thr) in enumerate(picks):
+            delay = t_start + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            eng.submit(Trial(x, banks[key[1]], thr, 2, tag=i))
+        th.join(timeout=120)
+        t_end = time.perf_counter()
+        assert len(results) == n_req, f"served {len(results)} of {n_req}"
+        lat = [r.latency_ms for r in results]
+        served = {"requests": n_req, "throughput": round(n_req / (t_end - t_start), 2), "p50_ms": pct(lat, 50),
+                  "p99_ms": pct(lat, 99), "statuses_ok": all(r.status == 0 for r in results)}
+        # plain form on the same arrivals: FIFO, one request per launch
+        plat = []
+        t_start = time.perf_counter()
+        for i, (key, x, thr) in enumerate(picks):
+            arr = t_start + arrivals[i]
+            delay = arr - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            plain_one(key, x, thr)
+            plat.append((time.perf_counter() - arr) * 1000.0)
+        t_end = time.perf_counter()
+        plain = {"throughput": round(n_req / (t_end - t_start), 2), "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99)}
+        out["rates"][str(int(rate))] = {"served": served, "plain": plain}
+        log(f"serve {rate:.0f} req/s: served {served} | plain {plain}")
+    st = eng.stats()
+    out["engine_stats"] = {k: int(v) for k, v in st.items()}
+    for f in plain_ctx.values():
+        f.close()
+    return out
+
+
+# --------------------------------------------------------------------------- main
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed batches of the batched form")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=64, help="trials per batch")
+    ap.add_argument("--forms", default="single,plain,batched,uneven,serve",
+                    help="comma list of single, plain, batched, uneven, serve")
+    ap.add_argument("--tile", type=int, default=0, help="engine tile size (0: autotuned per shape)")
+    ap.add_argument("--rows-per-item", type=int, default=0)
+    ap.add_argument("--hsum-rows", type=int, default=0)
+    ap.add_argument("--group-trials", type=int, default=8)
+    ap.add_argument("--ftc-wg-per-cu", type=int, default=0,
+                    help="FT-convolution workgroups resident per CU (0: one per work item)")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--plain-trials", type=int, default=20)
+    ap.add_argument("--serve-rates", default="50,100,200,400")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the FDAS hot path shards DM trials across GPUs with no "
+              "collective, so a multi-GPU figure is N independent copies of this run (see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    forms = set(args.forms.split(","))
+
+    import fdas_amd
+    from fdas_amd import synth
+
+    w = synth.full_workload()
+    log(f"workload: {w.name}")
+    x = w.spectrum()
+    bank85 = fdas_amd.make_drift_bank(synth.drift_grid(85), 421)
+    bank169 = fdas_amd.make_drift_bank(synth.drift_grid(169), 421)
+    thr = synth.noise_thresholds(x, H_METRIC)
+
+    out_forms = single_trial_forms(args, x, bank85, thr, forms) if forms & {"single", "plain"} else {}
+
+    batched = uneven = served = None
+    eng = None
+    if forms & {"batched", "uneven", "serve"}:
+        eng = make_engine(args)
+        banks = {85: eng.add_bank(bank85), 169: eng.add_bank(bank169)}
+        if "batched" in forms:
+            batched = batched_form(args, eng, banks[85])
+        if "uneven" in forms:
+            uneven = uneven_form(args, eng, banks)
+        if "serve" in forms:
+            served = served_form(args, eng, banks, {85: bank85, 169: bank169})
+        eng.close()
+
+    if batched is not None:
+        value, ms_per_step = batched["value"], batched["ms_per_step"]
+        steps, tile, trials_per_launch = args.steps, batched["tile"], args.group_trials
+        kernels = {
+            f"ftconv_kernel<{tile}>": {"ms": batched["ftc_span_ms"],
+                                       "bytes": ftc_bytes(N_METRIC, T_METRIC, tile, trials_per_launch)},
+            "hsum_kernel<8,true>": {"ms": batched["hsum_span_ms"],
+                                    "bytes": hsum_bytes(N_METRIC, T_METRIC,
+                                                        trials_per_launch * batched["candidates_per_trial_mean"],
+                                                        trials_per_launch)},
+        }
+        workload = (f"batched uniform: {args.batch} DM trials per step, N=2^21 bins, 85 templates x 421 taps, "
+                    f"8 harmonics, 4096 candidates/harmonic, spectra resident, groups of {args.group_trials}")
+        in_flight = f"2 batches, groups of {args.group_trials} trials"
+    else:
+        st = out_forms.get("single_trial_form")
+        if st is None:
+            raise SystemExit("bench.py: --forms needs batched or single for the headline value")
+        value, ms_per_step, steps, tile, trials_per_launch = st["value"], st["ms_per_trial"], 30, 2048, 1
+        kernels = {"ftconv_kernel<2048>": {"ms": st["ftconv_ms"], "bytes": ftc_bytes(N_METRIC, T_METRIC, 2048)},
+                   "hsum_kernel<8,true>": {"ms": st["hsum_ms"], "bytes": hsum_bytes(N_METRIC, T_METRIC, 8 * CAP)}}
+        workload = "single-trial FDAS, N=2^21 bins, 85 templates x 421 taps, 8 harmonics, spectra resident"
+        in_flight = "2 trials (buffer sets A/B)"
+
     dominant = max(kernels, key=lambda k: kernels[k]["ms"])
     kd = kernels[dominant]
     achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9
+    ftc_name = [k for k in kernels if k.startswith("ftconv")][0]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": dominant, "kernel_ms": round(kd["ms"], 4), "algorithmic_bytes": int(kd["bytes"]),
+                "trials_per_launch": trials_per_launch,
                 "kernels": {k: {"ms": round(v["ms"], 4), "algorithmic_bytes": int(v["bytes"]),
                                 "gbs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in kernels.items()},
-                "ftconv_fft_tflops": round(fft_flops(w.n_bins, w.n_templates, args.tile, 420) / (ftc_ms * 1e-3) / 1e12,
-                                           2),
-                "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS,
-                "select_ms": round(sel_ms, 4), "detections_per_plane": [int(v) for v in det]}
+                "ftconv_fft_tflops": round(trials_per_launch * fft_flops(N_METRIC, T_METRIC, tile, 420)
+                                           / (kernels[ftc_name]["ms"] * 1e-3) / 1e12, 2),
+                "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS}
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (bounded sample)")
-        cpu = cpu_baseline(x, bank, thr, cap)
+        cpu = cpu_baseline(x, bank85, thr, CAP)
         log(f"cpu: {cpu}")
 
-    f.close()
+    plain = out_forms.get("plain_gpu_form")
     out = {
         "metric": "DM-trials/sec (2^21 bins, 85 tmpl, 8 harm)",
         "value": round(value, 2),
         "unit": "DM-trials/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded N(0,1) series + drifting 8-harmonic tone; 85-template drift bank z=2(j-42))",
-        "config": {"workload": "single-trial FDAS, N=2^21 bins, 85 templates x 421 taps, FFT 2048 overlap-save, "
-                               "8 harmonics, 4096 candidates/harmonic, spectra resident",
-                   "n_bins": w.n_bins, "n_templates": w.n_templates, "tile": args.tile,
-                   "n_harmonics": w.n_harmonics, "cands_per_harmonic": cap, "in_flight": args.in_flight},
-        "isolated_form": isolated,
+        "data": "synthetic (seeded SplitMix64 N(0,1) series + 0-2 drifting 8-harmonic tones per trial; drift banks "
+                "z=2(j-c) of 85/169 templates)",
+        "config": {"workload": workload, "n_bins": N_METRIC, "n_templates": T_METRIC, "tile": tile,
+                   "n_harmonics": H_METRIC, "cands_per_harmonic": CAP, "batch": args.batch if batched else 1,
+                   "in_flight": in_flight},
+        "vs_plain": round(value / plain["value"], 2) if plain else None,
+        "batched_form": batched,
+        "single_trial_form": out_forms.get("single_trial_form"),
         "plain_gpu_form": plain,
+        "uneven_form": uneven,
+        "served_form": served,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

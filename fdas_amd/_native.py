@@ -39,6 +39,84 @@ class FdasCandidate(C.Structure):
     _fields_ = [("harmonic", C.c_uint32), ("template_num", C.c_int32), ("bin", C.c_uint32), ("power", C.c_float)]
 
 
+MAX_HARMONICS = 16
+REQ_DEVICE_SPECTRUM = 0x1
+PLAN_GRAPH = 0x1
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [
+        ("tile", C.c_uint32),
+        ("rows_per_item", C.c_uint32),
+        ("hsum_rows", C.c_uint32),
+        ("cands_per_harmonic", C.c_uint32),
+        ("spill_per_harmonic", C.c_uint32),
+        ("group_trials", C.c_uint32),
+        ("groups_in_flight", C.c_uint32),
+        ("device", C.c_int32),
+        ("hbm_budget_bytes", C.c_uint64),
+        ("ftc_workgroups_per_cu", C.c_uint32),
+        ("pad_", C.c_uint32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("spectrum", C.c_void_p),
+        ("n_bins", C.c_uint32),
+        ("bank", C.c_uint32),
+        ("n_harmonics", C.c_uint32),
+        ("part", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("pad_", C.c_uint32),
+        ("thresholds", C.c_float * MAX_HARMONICS),
+        ("tag", C.c_uint64),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("key", C.POINTER(C.c_uint32)),
+        ("power", C.POINTER(C.c_float)),
+        ("tag", C.c_uint64),
+        ("status", C.c_int32),
+        ("n_harmonics", C.c_uint32),
+        ("n_detected", C.c_uint32 * MAX_HARMONICS),
+        ("n_kept", C.c_uint32 * MAX_HARMONICS),
+        ("tile", C.c_uint32),
+        ("pad_", C.c_uint32),
+        ("latency_ms", C.c_double),
+    ]
+
+
+class EngineStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("groups_launched", C.c_uint64),
+        ("arena_reserved_bytes", C.c_uint64),
+        ("arena_in_use_bytes", C.c_uint64),
+        ("arena_peak_bytes", C.c_uint64),
+        ("queued", C.c_uint32),
+        ("in_flight", C.c_uint32),
+        ("max_group", C.c_uint32),
+        ("pad_", C.c_uint32),
+    ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [
+        ("n_trials", C.c_uint32),
+        ("n_groups", C.c_uint32),
+        ("n_launches", C.c_uint32),
+        ("graph", C.c_uint32),
+        ("region_bytes", C.c_uint64),
+        ("last_ms", C.c_float),
+        ("ftc_ms", C.c_float),
+        ("hsum_ms", C.c_float),
+    ]
+
+
 LOG_FN = C.CFUNCTYPE(None, C.c_int, C.c_char_p, C.c_void_p)
 
 _P = C.c_void_p
@@ -81,6 +159,21 @@ SIGNATURES = {
     "fdas_status_string": (C.c_char_p, [_I]),
     "fdas_make_drift_bank": (_I, [_D, _u32, _u32, _F]),
     "fdas_reference_drifts": (_I, [_u32, C.c_double, C.c_double, C.c_double, C.c_double, _D]),
+    "fdas_engine_default_config": (None, [C.POINTER(EngineConfig)]),
+    "fdas_engine_create": (_I, [C.POINTER(EngineConfig), LOG_FN, _P, C.POINTER(_P)]),
+    "fdas_engine_destroy": (None, [_P]),
+    "fdas_engine_last_error": (C.c_char_p, [_P]),
+    "fdas_engine_add_bank": (_I, [_P, _F, _u32, _u32, _U]),
+    "fdas_engine_prepare": (_I, [_P, _u32, _u32, _u32, _U]),
+    "fdas_plan_create": (_I, [_P, C.POINTER(Request), _u32, _u32, C.POINTER(_P)]),
+    "fdas_plan_set_request": (_I, [_P, _u32, _P, _F]),
+    "fdas_plan_launch": (_I, [_P]),
+    "fdas_plan_wait": (_I, [_P, C.POINTER(Result)]),
+    "fdas_plan_get_info": (_I, [_P, C.POINTER(PlanInfo)]),
+    "fdas_plan_destroy": (None, [_P]),
+    "fdas_engine_submit": (_I, [_P, C.POINTER(Request), C.POINTER(Result)]),
+    "fdas_engine_poll": (_I, [_P, C.POINTER(C.POINTER(Result)), _u32, _U, C.c_int32]),
+    "fdas_engine_get_stats": (_I, [_P, C.POINTER(EngineStats)]),
 }
 
 _lib = None
@@ -94,6 +187,13 @@ def load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m fdas_amd.build` "
                            "(the FDAS device path has no CPU fallback)")
+    # PyTorch ships its own HIP runtime (same soname as ROCm's).  Loading it first
+    # makes this library bind to that one runtime; loading ROCm's first would put
+    # two HIP runtimes in the process, and the second one finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

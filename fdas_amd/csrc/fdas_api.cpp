@@ -27,7 +27,6 @@ using namespace fdas;
 
 namespace {
 
-constexpr uint32_t kDefaultSpill = 262144;
 constexpr uint32_t kMaxFtcJobs = 64;  // row groups per FT-convolution launch
 
 // Events per launch: START, INPUT (spectrum in place), FTC_K (FT-convolution kernel start), FTC (its end),
@@ -437,7 +436,7 @@ extern "C" fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void
   }
   ctx->geo = make_geometry(cfg->n_bins, cfg->tile, cfg->max_taps);
   ctx->c = cfg->n_templates / 2;
-  ctx->spill = cfg->spill_per_harmonic ? cfg->spill_per_harmonic : kDefaultSpill;
+  ctx->spill = cfg->spill_per_harmonic ? cfg->spill_per_harmonic : default_spill(cfg->n_templates, cfg->n_bins);
   ctx->sync_pipeline = (cfg->flags & FDAS_FLAG_SYNC_PIPELINE) != 0;
 
   auto bail = [&](fdas_status s) {

@@ -1,0 +1,1450 @@
+// Batched and served FDAS behind the engine / plan entry points of include/fdas.h.
+//
+// The reference runs one trial per launch on two fixed buffer sets
+// (FDAS.cpp:282-615).  Here an engine keeps banks, twiddles and a device arena
+// bounded by an HBM budget; trials of any N, bank, harmonic count and part are
+// cut into groups.  One group's FT convolution runs on the convolution stream
+// while the previous group's harmonic summing, selection and read-back run on
+// the summing stream, so the VALU-bound convolution overlaps the HBM-bound
+// summing.  The same group machinery serves fixed batches (plans, optionally
+// captured once as a HIP graph) and a continuous-batching request loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "fdas_geometry.h"
+#include "fdas_host.h"
+#include "fdas_jobs.h"
+#include "fdas_kernels.h"
+#include "fdas_plan.h"
+
+using namespace fdas;
+
+namespace {
+
+constexpr uint32_t kTileSizes[3] = {1024, 2048, 4096};
+constexpr uint32_t kMaxRowGroups = 64;  // same cap as the single-trial path, so both plan identical row groups
+constexpr uint64_t kAlign = 256;
+constexpr uint64_t kArenaChunk = 4ull << 30;
+
+uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit sub-allocator over device chunks; the chunks are reserved lazily and
+// never exceed the budget in total.
+class Arena {
+ public:
+  explicit Arena(uint64_t budget) : budget_(budget) {}
+  ~Arena() { release(); }
+
+  void release() {
+    for (auto& c : chunks_) (void)hipFree(c.base);
+    chunks_.clear();
+    live_.clear();
+    reserved_ = in_use_ = 0;
+  }
+
+  // nullptr when `bytes` cannot be placed within the budget (err: the HIP error, if any).
+  char* alloc(uint64_t bytes, hipError_t* err) {
+    bytes = align_up(std::max<uint64_t>(bytes, 1));
+    for (size_t ci = 0; ci < chunks_.size(); ++ci)
+      if (char* p = take(ci, bytes)) return p;
+    if (reserved_ + bytes > budget_) return nullptr;
+    const uint64_t size = std::min(budget_ - reserved_, std::max(bytes, kArenaChunk));
+    void* base = nullptr;
+    const hipError_t e = hipMalloc(&base, size);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (err) *err = e;
+      return nullptr;
+    }
+    chunks_.push_back(Chunk{static_cast<char*>(base), size, {{0, size}}});
+    reserved_ += size;
+    return take(chunks_.size() - 1, bytes);
+  }
+
+  void free(void* p) {
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    Chunk& c = chunks_[it->second.first];
+    uint64_t off = uint64_t(static_cast<char*>(p) - c.base), size = it->second.second;
+    in_use_ -= size;
+    live_.erase(it);
+    auto nx = c.free.lower_bound(off);
+    if (nx != c.free.end() && off + size == nx->first) {
+      size += nx->second;
+      nx = c.free.erase(nx);
+    }
+    if (nx != c.free.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        pv->second += size;
+        return;
+      }
+    }
+    c.free.emplace(off, size);
+  }
+
+  uint64_t reserved() const { return reserved_; }
+  uint64_t in_use() const { return in_use_; }
+  uint64_t peak() const { return peak_; }
+
+ private:
+  struct Chunk {
+    char* base;
+    uint64_t size;
+    std::map<uint64_t, uint64_t> free;  // offset -> size
+  };
+
+  char* take(size_t ci, uint64_t bytes) {
+    Chunk& c = chunks_[ci];
+    for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+      if (it->second < bytes) continue;
+      const uint64_t off = it->first, rest = it->second - bytes;
+      c.free.erase(it);
+      if (rest) c.free.emplace(off + bytes, rest);
+      char* p = c.base + off;
+      live_[p] = {ci, bytes};
+      in_use_ += bytes;
+      peak_ = std::max(peak_, in_use_);
+      return p;
+    }
+    return nullptr;
+  }
+
+  uint64_t budget_, reserved_ = 0, in_use_ = 0, peak_ = 0;
+  std::vector<Chunk> chunks_;
+  std::unordered_map<void*, std::pair<size_t, uint64_t>> live_;
+};
+
+struct Bank {
+  uint32_t n_templates = 0, n_taps = 0, max_extent = 1;
+  std::vector<float> taps;        // T x n_taps complex
+  std::vector<uint32_t> extent;   // non-zero taps per row
+  std::map<uint32_t, float2*> spectra;  // tile -> [T][16][S/16] kernel layout, 1/S scaled
+};
+
+// Work shape of (N, bank, part), fixed once: the tile decides the FOP rounding,
+// so a shape keeps it for the engine's lifetime.
+struct Shape {
+  uint32_t tile = 2048, rows_per_item = 16, hsum_rows = kHsumRowsPerBlock;
+  uint32_t row_begin = 0, n_rows = 0;
+  std::vector<FtcJob> ftc;  // row groups, pointers unset, item_begin relative
+  uint32_t n_items = 0;
+};
+
+// Byte offsets of one trial's device state inside its arena block.  The first
+// four arrays are contiguous so one copy reads the lists back; out_n .. hist is
+// zeroed per run.
+struct TrialLayout {
+  uint64_t out_n, counts, out_key, out_pwr, compact_n, hist, compact_key, compact_pwr, spill_key, spill_pwr, input,
+      fop, total;
+  uint64_t readback_bytes;
+  uint32_t zero_words;
+};
+
+TrialLayout trial_layout(uint32_t n_bins, uint32_t T, uint32_t H, uint32_t cap, uint32_t spill, bool own_input) {
+  TrialLayout l{};
+  uint64_t o = 0;
+  l.out_n = o;
+  o += 4ull * H;
+  l.counts = o;
+  o += 4ull * H;
+  l.out_key = o;
+  o += 4ull * H * cap;
+  l.out_pwr = o;
+  o += 4ull * H * cap;
+  l.readback_bytes = o;
+  l.compact_n = o;
+  o += 4ull * H;
+  l.hist = o;
+  o += 4ull * H * kSelectBins;
+  l.zero_words = uint32_t(o / 4);
+  const uint64_t ccap = uint64_t(cap) + kSelectPool;
+  o = align_up(o);
+  l.compact_key = o;
+  o = align_up(o + 4ull * H * ccap);
+  l.compact_pwr = o;
+  o = align_up(o + 4ull * H * ccap);
+  l.spill_key = o;
+  o = align_up(o + 4ull * H * spill);
+  l.spill_pwr = o;
+  o = align_up(o + 4ull * H * spill);
+  l.input = o;
+  if (own_input) o = align_up(o + 8ull * n_bins);
+  l.fop = o;
+  o = align_up(o + 4ull * T * n_bins);
+  l.total = o;
+  return l;
+}
+
+// One trial placed in a group.
+struct Trial {
+  fdas_request req{};
+  fdas_result* res = nullptr;     // served trials: caller's result
+  const Shape* shape = nullptr;
+  const Bank* bank = nullptr;
+  TrialLayout lay{};
+  uint32_t spill = 0;             // spill capacity per plane
+  char* mem = nullptr;            // device block (arena)
+  const float2* input = nullptr;  // device spectrum
+  char* h_read = nullptr;         // pinned read-back area (lay.readback_bytes)
+  double t_submit = 0.0;
+};
+
+// Device and pinned job arrays of one group, and its launch list.
+struct GroupBuf {
+  FtcJob* d_ftc = nullptr;
+  FtcJob* h_ftc = nullptr;
+  uint32_t ftc_cap = 0;
+  HsumJob* d_hsum = nullptr;
+  HsumJob* h_hsum = nullptr;
+  uint32_t hsum_cap = 0;
+  SelectJob* d_sel = nullptr;
+  SelectJob* h_sel = nullptr;
+  uint32_t sel_cap = 0;
+  hipEvent_t ev_ftc0 = nullptr, ev_ftc1 = nullptr, ev_hsum0 = nullptr, ev_hsum1 = nullptr, ev_done = nullptr;
+  struct FtcLaunch {
+    uint32_t tile, first, n_jobs, n_items;
+  };
+  struct HsumLaunch {
+    uint32_t H, first, n_jobs, n_blocks;
+    bool vec;
+  };
+  std::vector<FtcLaunch> ftc;
+  std::vector<HsumLaunch> hsum;
+  uint32_t n_sel = 0;
+};
+
+}  // namespace
+
+struct fdas_engine {
+  fdas_engine_config cfg{};
+  fdas_log_fn log = nullptr;
+  void* user = nullptr;
+  std::string last_error;
+  std::mutex err_mu;  // last_error is written by API calls and by the scheduler thread
+  std::mutex api_mu;  // banks, shapes, arena and streams (held by the scheduler while it launches)
+  std::unique_ptr<Arena> arena;
+  std::vector<std::unique_ptr<Bank>> banks;
+  std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<Shape>> shapes;
+  std::map<uint32_t, std::pair<float2*, float2*>> twiddles;  // tile -> (tw1, tw2)
+  hipStream_t s_ftc = nullptr, s_hsum = nullptr;
+  float2* zero_input = nullptr;  // 2^22 zero bins (tuning input)
+  uint32_t n_cu = 0;
+  uint32_t zero_input_bins = 0;
+
+  // serving state (guarded by q_mu)
+  std::mutex q_mu;
+  std::condition_variable q_cv, done_cv;
+  std::deque<Trial> queue;
+  std::deque<fdas_result*> done;
+  std::thread scheduler;
+  bool stop = false, running = false;
+  struct Ctx {
+    GroupBuf buf;
+    std::vector<Trial> trials;
+    char* h_read = nullptr;
+    uint64_t h_read_cap = 0;
+    bool busy = false;
+    uint64_t seq = 0;
+  };
+  std::vector<Ctx> ctxs;
+  uint64_t launch_seq = 0;
+  fdas_engine_stats stats{};
+};
+
+struct fdas_plan {
+  fdas_engine* eng = nullptr;
+  std::vector<Trial> trials;
+  std::vector<std::vector<uint32_t>> groups;
+  std::vector<GroupBuf> bufs;
+  std::vector<char*> regions;
+  std::vector<char*> inputs;  // plan-owned device copies of host spectra (nullptr for device spectra)
+  char* h_read = nullptr;
+  uint64_t region_bytes = 0;
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = false, launched = false, eager_timed = false;
+  uint32_t n_launches = 0;
+};
+
+namespace {
+
+void emit(fdas_engine* e, int level, const std::string& msg) {
+  if (e && e->log) e->log(level, msg.c_str(), e->user);
+}
+
+fdas_status fail(fdas_engine* e, fdas_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (e) {
+    const std::string msg = std::string("[ERROR] ") + buf;
+    {
+      std::lock_guard<std::mutex> lk(e->err_mu);
+      e->last_error = msg;
+    }
+    emit(e, FDAS_LOG_ERROR, msg);
+  }
+  return st;
+}
+
+void info(fdas_engine* e, const char* fmt, ...) {
+  if (!e || !e->log) return;
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  emit(e, FDAS_LOG_INFO, std::string("[INFO] ") + buf);
+}
+
+#define EHIP(eng, expr)                                                                                        \
+  do {                                                                                                         \
+    hipError_t e_ = (expr);                                                                                    \
+    if (e_ != hipSuccess)                                                                                      \
+      return fail(eng, FDAS_EDEVICE, "HIP call failed with status %d (%s):\n          %s [%s:%d]", int(e_), \
+                  hipGetErrorString(e_), #expr, __FILE__, __LINE__);                                           \
+  } while (0)
+
+// Workgroups of an FT-convolution launch over n_items items.
+uint32_t ftc_grid(const fdas_engine* e, uint32_t n_items) {
+  const uint32_t cap = e->cfg.ftc_workgroups_per_cu * e->n_cu;
+  return cap ? std::min(n_items, cap) : n_items;
+}
+
+void part_rows(uint32_t T, uint32_t part, uint32_t* begin, uint32_t* count) {
+  const uint32_t c = T / 2;
+  switch (part) {
+    case FDAS_NEGATIVE: *begin = 0; *count = c + 1; break;
+    case FDAS_POSITIVE: *begin = c; *count = c + 1; break;
+    default: *begin = 0; *count = T; break;
+  }
+}
+
+bool tile_usable(uint32_t tile, uint32_t max_extent) { return 2 * overlap_for(max_extent) <= tile; }
+
+template <typename T>
+fdas_status dev_alloc(fdas_engine* e, T** p, size_t count) {
+  *p = nullptr;
+  const hipError_t r = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+  if (r != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(e, FDAS_ENOMEM, "device allocation of %zu bytes failed (%s)", count * sizeof(T), hipGetErrorString(r));
+  }
+  return FDAS_OK;
+}
+
+template <typename T>
+fdas_status host_alloc(fdas_engine* e, T** p, size_t count) {
+  *p = nullptr;
+  const hipError_t r = hipHostMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T), 0);
+  if (r != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(e, FDAS_ENOMEM, "pinned host allocation of %zu bytes failed (%s)", count * sizeof(T),
+                hipGetErrorString(r));
+  }
+  return FDAS_OK;
+}
+
+void free_group(GroupBuf& g) {
+  (void)hipFree(g.d_ftc);
+  (void)hipFree(g.d_hsum);
+  (void)hipFree(g.d_sel);
+  (void)hipHostFree(g.h_ftc);
+  (void)hipHostFree(g.h_hsum);
+  (void)hipHostFree(g.h_sel);
+  for (hipEvent_t ev : {g.ev_ftc0, g.ev_ftc1, g.ev_hsum0, g.ev_hsum1, g.ev_done})
+    if (ev) (void)hipEventDestroy(ev);
+  g = GroupBuf{};
+}
+
+fdas_status group_events(fdas_engine* e, GroupBuf& g) {
+  for (hipEvent_t* ev : {&g.ev_ftc0, &g.ev_ftc1, &g.ev_hsum0, &g.ev_hsum1, &g.ev_done})
+    if (!*ev) EHIP(e, hipEventCreate(ev));
+  return FDAS_OK;
+}
+
+// Grows the job arrays of g to hold at least the given counts.
+fdas_status group_reserve(fdas_engine* e, GroupBuf& g, uint32_t n_ftc, uint32_t n_hsum, uint32_t n_sel) {
+  fdas_status st;
+  if (n_ftc > g.ftc_cap) {
+    (void)hipFree(g.d_ftc);
+    (void)hipHostFree(g.h_ftc);
+    g.d_ftc = nullptr;
+    g.h_ftc = nullptr;
+    g.ftc_cap = 0;
+    if ((st = dev_alloc(e, &g.d_ftc, n_ftc)) != FDAS_OK || (st = host_alloc(e, &g.h_ftc, n_ftc)) != FDAS_OK) return st;
+    g.ftc_cap = n_ftc;
+  }
+  if (n_hsum > g.hsum_cap) {
+    (void)hipFree(g.d_hsum);
+    (void)hipHostFree(g.h_hsum);
+    g.d_hsum = nullptr;
+    g.h_hsum = nullptr;
+    g.hsum_cap = 0;
+    if ((st = dev_alloc(e, &g.d_hsum, n_hsum)) != FDAS_OK || (st = host_alloc(e, &g.h_hsum, n_hsum)) != FDAS_OK)
+      return st;
+    g.hsum_cap = n_hsum;
+  }
+  if (n_sel > g.sel_cap) {
+    (void)hipFree(g.d_sel);
+    (void)hipHostFree(g.h_sel);
+    g.d_sel = nullptr;
+    g.h_sel = nullptr;
+    g.sel_cap = 0;
+    if ((st = dev_alloc(e, &g.d_sel, n_sel)) != FDAS_OK || (st = host_alloc(e, &g.h_sel, n_sel)) != FDAS_OK) return st;
+    g.sel_cap = n_sel;
+  }
+  return group_events(e, g);
+}
+
+// Fills the host job arrays of a group of placed trials (mem, input, h_read set)
+// and its launch list: FT-convolution jobs grouped by tile, summing jobs by H.
+fdas_status build_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& trials) {
+  const uint32_t cap = e->cfg.cands_per_harmonic, ccap = cap + kSelectPool;
+  uint32_t n_ftc = 0, n_sel = 0;
+  for (const Trial* t : trials) {
+    n_ftc += uint32_t(t->shape->ftc.size());
+    n_sel += t->req.n_harmonics;
+  }
+  fdas_status st = group_reserve(e, g, n_ftc, uint32_t(trials.size()), n_sel);
+  if (st != FDAS_OK) return st;
+  g.ftc.clear();
+  g.hsum.clear();
+  uint32_t jf = 0;
+  for (uint32_t tile : kTileSizes) {
+    GroupBuf::FtcLaunch L{tile, jf, 0, 0};
+    for (const Trial* t : trials) {
+      if (t->shape->tile != tile) continue;
+      float* fop = reinterpret_cast<float*>(t->mem + t->lay.fop);
+      for (FtcJob j : t->shape->ftc) {
+        j.spectrum = t->input;
+        j.fop = fop;
+        j.hperm = t->bank->spectra.at(tile);
+        j.item_begin = L.n_items;
+        L.n_items += j.n_tiles * j.n_chunks;
+        g.h_ftc[jf++] = j;
+      }
+    }
+    L.n_jobs = jf - L.first;
+    if (L.n_jobs) g.ftc.push_back(L);
+  }
+  uint32_t jh = 0, js = 0;
+  for (uint32_t H = 1; H <= kMaxHarmonics; ++H) {
+    GroupBuf::HsumLaunch L{H, jh, 0, 0, true};
+    for (const Trial* t : trials) {
+      if (t->req.n_harmonics != H) continue;
+      char* m = t->mem;
+      const TrialLayout& l = t->lay;
+      const uint32_t spill = t->spill;
+      HsumJob& j = g.h_hsum[jh++];
+      j = HsumJob{};
+      j.fop = reinterpret_cast<const float*>(m + l.fop);
+      j.n_bins = t->req.n_bins;
+      j.n_templates = t->bank->n_templates;
+      j.row_begin = t->shape->row_begin;
+      j.n_rows = t->shape->n_rows;
+      j.n_harmonics = H;
+      j.spill_cap = spill;
+      j.block_begin = L.n_blocks;
+      hsum_row_chunks(j.n_rows, &j.rows_per_block, &j.n_row_chunks, t->shape->hsum_rows);
+      j.spill_key = reinterpret_cast<uint32_t*>(m + l.spill_key);
+      j.spill_pwr = reinterpret_cast<float*>(m + l.spill_pwr);
+      j.counts = reinterpret_cast<uint32_t*>(m + l.counts);
+      for (uint32_t k = 0; k < H; ++k) j.thr[k] = t->req.thresholds[k];
+      L.n_blocks += (j.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
+      L.vec = L.vec && (j.n_bins & 3u) == 0;
+      for (uint32_t p = 0; p < H; ++p) {
+        SelectJob& q = g.h_sel[js++];
+        q.spill_key = j.spill_key + size_t(p) * spill;
+        q.spill_pwr = j.spill_pwr + size_t(p) * spill;
+        q.count = j.counts + p;
+        q.spill_cap = spill;
+        q.cap = cap;
+        q.thr = t->req.thresholds[p];
+        q.compact_cap = ccap;
+        q.hist = reinterpret_cast<uint32_t*>(m + l.hist) + size_t(p) * kSelectBins;
+        q.compact_key = reinterpret_cast<uint32_t*>(m + l.compact_key) + size_t(p) * ccap;
+        q.compact_pwr = reinterpret_cast<float*>(m + l.compact_pwr) + size_t(p) * ccap;
+        q.compact_n = reinterpret_cast<uint32_t*>(m + l.compact_n) + p;
+        q.out_key = reinterpret_cast<uint32_t*>(m + l.out_key) + size_t(p) * cap;
+        q.out_pwr = reinterpret_cast<float*>(m + l.out_pwr) + size_t(p) * cap;
+        q.out_n = reinterpret_cast<uint32_t*>(m + l.out_n) + p;
+      }
+    }
+    L.n_jobs = jh - L.first;
+    if (L.n_jobs) g.hsum.push_back(L);
+  }
+  g.n_sel = js;
+  (void)n_sel;
+  return FDAS_OK;
+}
+
+fdas_status upload_group(fdas_engine* e, GroupBuf& g, uint32_t n_ftc, uint32_t n_hsum, hipStream_t s) {
+  EHIP(e, hipMemcpyAsync(g.d_ftc, g.h_ftc, sizeof(FtcJob) * n_ftc, hipMemcpyHostToDevice, s));
+  EHIP(e, hipMemcpyAsync(g.d_hsum, g.h_hsum, sizeof(HsumJob) * n_hsum, hipMemcpyHostToDevice, s));
+  EHIP(e, hipMemcpyAsync(g.d_sel, g.h_sel, sizeof(SelectJob) * g.n_sel, hipMemcpyHostToDevice, s));
+  return FDAS_OK;
+}
+
+uint32_t group_ftc_jobs(const GroupBuf& g) {
+  uint32_t n = 0;
+  for (const auto& L : g.ftc) n += L.n_jobs;
+  return n;
+}
+
+uint32_t group_hsum_jobs(const GroupBuf& g) {
+  uint32_t n = 0;
+  for (const auto& L : g.hsum) n += L.n_jobs;
+  return n;
+}
+
+// Enqueues one group: convolution on s_ftc, then (after its end event) zeroing,
+// summing, selection and read-back on s_hsum.  wait_before: an event s_ftc waits
+// on first (the group whose memory this one reuses), or nullptr.
+fdas_status enqueue_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& trials, hipEvent_t wait_before,
+                          bool timing_events) {
+  hipStream_t A = e->s_ftc, B = e->s_hsum;
+  if (wait_before) EHIP(e, hipStreamWaitEvent(A, wait_before, 0));
+  if (timing_events) EHIP(e, hipEventRecord(g.ev_ftc0, A));
+  for (const auto& L : g.ftc) {
+    const auto& tw = e->twiddles.at(L.tile);
+    EHIP(e, launch_ftconv(int(L.tile), g.d_ftc + L.first, L.n_jobs, L.n_items, tw.first, tw.second,
+                          ftc_grid(e, L.n_items), A));
+  }
+  EHIP(e, hipEventRecord(g.ev_ftc1, A));
+  EHIP(e, hipStreamWaitEvent(B, g.ev_ftc1, 0));
+  for (const Trial* t : trials)
+    EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(t->mem + t->lay.out_n), t->lay.zero_words, B));
+  if (timing_events) EHIP(e, hipEventRecord(g.ev_hsum0, B));
+  for (const auto& L : g.hsum) EHIP(e, launch_hsum(g.d_hsum + L.first, L.n_jobs, L.n_blocks, L.H, L.vec, B));
+  if (timing_events) EHIP(e, hipEventRecord(g.ev_hsum1, B));
+  EHIP(e, launch_select(g.d_sel, g.n_sel, B));
+  for (const Trial* t : trials)
+    EHIP(e, hipMemcpyAsync(t->h_read, t->mem + t->lay.out_n, t->lay.readback_bytes, hipMemcpyDeviceToHost, B));
+  EHIP(e, hipEventRecord(g.ev_done, B));
+  return FDAS_OK;
+}
+
+// Copies a completed trial's lists from its pinned read-back area to the result.
+void finish_result(const fdas_engine* e, const Trial& t, fdas_result* r) {
+  const uint32_t H = t.req.n_harmonics, cap = e->cfg.cands_per_harmonic;
+  const uint32_t* out_n = reinterpret_cast<const uint32_t*>(t.h_read + t.lay.out_n);
+  const uint32_t* counts = reinterpret_cast<const uint32_t*>(t.h_read + t.lay.counts);
+  const uint32_t* key = reinterpret_cast<const uint32_t*>(t.h_read + t.lay.out_key);
+  const float* pwr = reinterpret_cast<const float*>(t.h_read + t.lay.out_pwr);
+  r->tag = t.req.tag;
+  r->n_harmonics = H;
+  r->tile = t.shape->tile;
+  r->status = FDAS_OK;
+  for (uint32_t p = 0; p < kMaxHarmonics; ++p) {
+    r->n_detected[p] = p < H ? counts[p] : 0;
+    r->n_kept[p] = p < H ? std::min(out_n[p], cap) : 0;
+    if (p < H && counts[p] > t.spill) r->status = FDAS_EOVERFLOW;
+  }
+  if (r->key) std::memcpy(r->key, key, sizeof(uint32_t) * H * cap);
+  if (r->power) std::memcpy(r->power, pwr, sizeof(float) * H * cap);
+}
+
+fdas_status validate_request(fdas_engine* e, const fdas_request* q) {
+  if (!q) return fail(e, FDAS_EINVAL, "null request");
+  if (!q->spectrum) return fail(e, FDAS_EINVAL, "request %llu: null spectrum", (unsigned long long)q->tag);
+  if (q->n_bins < 1 || q->n_bins > kMaxBins)
+    return fail(e, FDAS_EINVAL, "request %llu: n_bins %u outside 1 .. %u", (unsigned long long)q->tag, q->n_bins,
+                kMaxBins);
+  if (q->bank >= e->banks.size())
+    return fail(e, FDAS_EINVAL, "request %llu: unknown bank %u", (unsigned long long)q->tag, q->bank);
+  if (q->n_harmonics < 1 || q->n_harmonics > kMaxHarmonics)
+    return fail(e, FDAS_EINVAL, "request %llu: n_harmonics %u outside 1 .. %u", (unsigned long long)q->tag,
+                q->n_harmonics, kMaxHarmonics);
+  if (q->part > FDAS_ALL) return fail(e, FDAS_EINVAL, "request %llu: invalid part %u", (unsigned long long)q->tag, q->part);
+  for (uint32_t k = 0; k < q->n_harmonics; ++k)
+    if (std::isnan(q->thresholds[k]))
+      return fail(e, FDAS_EINVAL, "request %llu: threshold %u is NaN", (unsigned long long)q->tag, k + 1);
+  return FDAS_OK;
+}
+
+// Times one launch list (best of `reps`, after one warm-up) on s_ftc.
+template <typename F>
+fdas_status time_best(fdas_engine* e, F&& enqueue, int reps, float* best) {
+  hipEvent_t a, b;
+  EHIP(e, hipEventCreate(&a));
+  EHIP(e, hipEventCreate(&b));
+  *best = FLT_MAX;
+  fdas_status st = FDAS_OK;
+  for (int r = 0; r <= reps && st == FDAS_OK; ++r) {
+    if (hipEventRecord(a, e->s_ftc) != hipSuccess) st = FDAS_EDEVICE;
+    if (st == FDAS_OK) st = enqueue();
+    if (st == FDAS_OK && hipEventRecord(b, e->s_ftc) != hipSuccess) st = FDAS_EDEVICE;
+    if (st == FDAS_OK && hipEventSynchronize(b) != hipSuccess) st = FDAS_EDEVICE;
+    float ms = 0.f;
+    if (st == FDAS_OK && r > 0 && hipEventElapsedTime(&ms, a, b) == hipSuccess) *best = std::min(*best, ms);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (st == FDAS_EDEVICE) return fail(e, st, "HIP call failed while autotuning");
+  return st;
+}
+
+std::vector<FtcJob> shape_jobs(const Bank& bank, uint32_t tile, uint32_t n_bins, uint32_t rb, uint32_t nr,
+                               uint32_t rows_per_item, uint32_t* n_items) {
+  std::vector<FtcJob> jobs =
+      plan_ftc_rows(tile, n_bins, bank.n_taps, bank.extent, rb, nr, kMaxRowGroups, rows_per_item);
+  uint32_t item = 0;
+  for (FtcJob& j : jobs) {
+    j.item_begin = item;
+    item += j.n_tiles * j.n_chunks;
+  }
+  *n_items = item;
+  return jobs;
+}
+
+// Chooses (tile, rows per item, rows per summing workgroup) for a shape by timing
+// the kernels on a zero spectrum; caller holds api_mu and s_ftc is idle or may
+// be waited on.
+fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32_t part, Shape* out) {
+  const Bank& bank = *e->banks[bank_id];
+  const uint32_t T = bank.n_templates;
+  uint32_t rb, nr;
+  part_rows(T, part, &rb, &nr);
+  out->row_begin = rb;
+  out->n_rows = nr;
+  std::vector<uint32_t> tiles;
+  if (e->cfg.tile) {
+    tiles.push_back(e->cfg.tile);
+  } else {
+    for (uint32_t s : kTileSizes)
+      if (tile_usable(s, bank.max_extent)) tiles.push_back(s);
+  }
+  std::vector<uint32_t> rpis = e->cfg.rows_per_item ? std::vector<uint32_t>{e->cfg.rows_per_item}
+                                                    : std::vector<uint32_t>{8, 16, 32};
+  std::vector<uint32_t> hrows = e->cfg.hsum_rows ? std::vector<uint32_t>{e->cfg.hsum_rows}
+                                                 : std::vector<uint32_t>{6, kHsumRowsPerBlock, 22};
+  // Defaults used when nothing needs timing.
+  out->tile = tiles.size() == 1 ? tiles[0] : (std::find(tiles.begin(), tiles.end(), 2048u) != tiles.end() ? 2048u : tiles[0]);
+  out->rows_per_item = rpis.size() == 1 ? rpis[0] : 16;
+  out->hsum_rows = hrows.size() == 1 ? hrows[0] : kHsumRowsPerBlock;
+  const bool tune_ftc = tiles.size() > 1 || rpis.size() > 1, tune_hsum = hrows.size() > 1;
+  if (tune_ftc || tune_hsum) {
+    const uint32_t spill = e->cfg.spill_per_harmonic ? e->cfg.spill_per_harmonic : default_spill(T, n_bins);
+    const TrialLayout lay = trial_layout(n_bins, T, 8, e->cfg.cands_per_harmonic, spill, false);
+    hipError_t herr = hipSuccess;
+    char* mem = e->arena->alloc(lay.total, &herr);
+    if (!mem) {
+      info(e, "autotune of N=%u bank %u skipped: no room in the HBM budget; using tile %u", n_bins, bank_id,
+           out->tile);
+    } else {
+      GroupBuf g;
+      fdas_status st = group_reserve(e, g, kMaxRowGroups, 1, 0);
+      float best = FLT_MAX;
+      for (uint32_t tile : tiles) {
+        for (uint32_t rpi : rpis) {
+          if (st != FDAS_OK || !tune_ftc) break;
+          uint32_t n_items = 0;
+          std::vector<FtcJob> jobs = shape_jobs(bank, tile, n_bins, rb, nr, rpi, &n_items);
+          for (FtcJob& j : jobs) {
+            j.spectrum = e->zero_input;
+            j.fop = reinterpret_cast<float*>(mem + lay.fop);
+            j.hperm = bank.spectra.at(tile);
+          }
+          if (hipMemcpy(g.d_ftc, jobs.data(), sizeof(FtcJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            st = fail(e, FDAS_EDEVICE, "job upload failed while autotuning");
+            break;
+          }
+          const auto& tw = e->twiddles.at(tile);
+          float ms = FLT_MAX;
+          st = time_best(
+              e,
+              [&]() -> fdas_status {
+                EHIP(e, launch_ftconv(int(tile), g.d_ftc, uint32_t(jobs.size()), n_items, tw.first, tw.second,
+                                      ftc_grid(e, n_items), e->s_ftc));
+                return FDAS_OK;
+              },
+              2, &ms);
+          info(e, "autotune N=%u bank %u: tile %u, %u rows/item: %.4f ms", n_bins, bank_id, tile, rpi, ms);
+          if (st == FDAS_OK && ms < best) {
+            best = ms;
+            out->tile = tile;
+            out->rows_per_item = rpi;
+          }
+        }
+      }
+      if (st == FDAS_OK && tune_hsum) {
+        // The FOP of the last timed (or a fresh) convolution is zero; thresholds
+        // at FLT_MAX keep the lists empty, so only the streaming cost is timed.
+        HsumJob j{};
+        j.fop = reinterpret_cast<const float*>(mem + lay.fop);
+        j.n_bins = n_bins;
+        j.n_templates = T;
+        j.row_begin = rb;
+        j.n_rows = nr;
+        j.n_harmonics = 8;
+        j.spill_cap = spill;
+        j.spill_key = reinterpret_cast<uint32_t*>(mem + lay.spill_key);
+        j.spill_pwr = reinterpret_cast<float*>(mem + lay.spill_pwr);
+        j.counts = reinterpret_cast<uint32_t*>(mem + lay.counts);
+        for (uint32_t k = 0; k < kMaxHarmonics; ++k) j.thr[k] = FLT_MAX;
+        if (!tune_ftc) {
+          // The FOP block is uninitialised: clear it so the thresholds hold.
+          st = hipMemsetAsync(mem + lay.fop, 0, 4ull * T * n_bins, e->s_ftc) == hipSuccess
+                   ? FDAS_OK
+                   : fail(e, FDAS_EDEVICE, "FOP clear failed while autotuning");
+        }
+        float bestH = FLT_MAX;
+        for (uint32_t hr : hrows) {
+          if (st != FDAS_OK) break;
+          hsum_row_chunks(nr, &j.rows_per_block, &j.n_row_chunks, hr);
+          const uint32_t blocks = (n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
+          if (hipMemcpy(g.d_hsum, &j, sizeof(HsumJob), hipMemcpyHostToDevice) != hipSuccess) {
+            st = fail(e, FDAS_EDEVICE, "job upload failed while autotuning");
+            break;
+          }
+          float ms = FLT_MAX;
+          st = time_best(
+              e,
+              [&]() -> fdas_status {
+                EHIP(e, launch_zero_u32(j.counts, 1, e->s_ftc));
+                EHIP(e, launch_hsum(g.d_hsum, 1, blocks, 8, (n_bins & 3u) == 0, e->s_ftc));
+                return FDAS_OK;
+              },
+              2, &ms);
+          info(e, "autotune N=%u bank %u: %u rows per summing workgroup: %.4f ms", n_bins, bank_id, hr, ms);
+          if (st == FDAS_OK && ms < bestH) {
+            bestH = ms;
+            out->hsum_rows = hr;
+          }
+        }
+      }
+      free_group(g);
+      e->arena->free(mem);
+      if (st != FDAS_OK) return st;
+    }
+  }
+  out->ftc = shape_jobs(bank, out->tile, n_bins, rb, nr, out->rows_per_item, &out->n_items);
+  info(e, "shape N=%u bank %u part %u: tile %u, %u rows/item, %u rows/summing workgroup, %zu row groups", n_bins,
+       bank_id, part, out->tile, out->rows_per_item, out->hsum_rows, out->ftc.size());
+  return FDAS_OK;
+}
+
+// Shape of a request, tuning it on first use (caller holds api_mu).
+fdas_status get_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part, const Shape** out) {
+  const auto key = std::make_tuple(n_bins, bank, part);
+  auto it = e->shapes.find(key);
+  if (it == e->shapes.end()) {
+    auto sh = std::make_unique<Shape>();
+    const fdas_status st = tune_shape(e, n_bins, bank, part, sh.get());
+    if (st != FDAS_OK) return st;
+    it = e->shapes.emplace(key, std::move(sh)).first;
+  }
+  *out = it->second.get();
+  return FDAS_OK;
+}
+
+// own_input: a host spectrum gets its device copy inside the trial's block
+// (served trials); plans keep such copies apart, since groups reuse blocks.
+fdas_status make_trial(fdas_engine* e, const fdas_request& q, bool own_input, Trial* t) {
+  fdas_status st = validate_request(e, &q);
+  if (st != FDAS_OK) return st;
+  const Shape* sh = nullptr;
+  if ((st = get_shape(e, q.n_bins, q.bank, q.part, &sh)) != FDAS_OK) return st;
+  t->req = q;
+  t->shape = sh;
+  t->bank = e->banks[q.bank].get();
+  t->spill = e->cfg.spill_per_harmonic ? e->cfg.spill_per_harmonic : default_spill(t->bank->n_templates, q.n_bins);
+  t->lay = trial_layout(q.n_bins, t->bank->n_templates, q.n_harmonics, e->cfg.cands_per_harmonic, t->spill,
+                        own_input && (q.flags & FDAS_REQ_DEVICE_SPECTRUM) == 0);
+  return FDAS_OK;
+}
+
+// ------------------------------------------------------------------ serving
+
+void scheduler_loop(fdas_engine* e);
+
+fdas_status start_scheduler(fdas_engine* e) {
+  if (e->running) return FDAS_OK;
+  e->ctxs.resize(e->cfg.groups_in_flight);
+  e->stop = false;
+  e->running = true;
+  e->scheduler = std::thread(scheduler_loop, e);
+  return FDAS_OK;
+}
+
+// Completes a finished context: results out, memory back to the arena.
+void complete_ctx(fdas_engine* e, fdas_engine::Ctx& c) {
+  const double t = now_ms();
+  std::vector<fdas_result*> out;
+  {
+    std::lock_guard<std::mutex> lk(e->api_mu);
+    for (Trial& tr : c.trials) {
+      finish_result(e, tr, tr.res);
+      tr.res->latency_ms = t - tr.t_submit;
+      e->arena->free(tr.mem);
+      out.push_back(tr.res);
+    }
+  }
+  c.trials.clear();
+  c.busy = false;
+  std::lock_guard<std::mutex> lk(e->q_mu);
+  for (fdas_result* r : out) e->done.push_back(r);
+  e->stats.completed += out.size();
+  e->stats.in_flight -= uint32_t(out.size());
+  e->done_cv.notify_all();
+}
+
+void fail_trial(fdas_engine* e, Trial& t, fdas_status st) {
+  t.res->tag = t.req.tag;
+  t.res->status = st;
+  t.res->n_harmonics = t.req.n_harmonics;
+  t.res->latency_ms = now_ms() - t.t_submit;
+  std::memset(t.res->n_detected, 0, sizeof(t.res->n_detected));
+  std::memset(t.res->n_kept, 0, sizeof(t.res->n_kept));
+  std::lock_guard<std::mutex> lk(e->q_mu);
+  e->done.push_back(t.res);
+  e->stats.completed += 1;
+  e->done_cv.notify_all();
+}
+
+// Admits queued trials into a free context and launches them.  Returns true if
+// anything was admitted or failed (progress).
+bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight) {
+  std::vector<Trial> picked;
+  {
+    std::lock_guard<std::mutex> lk(e->q_mu);
+    if (e->queue.empty()) return false;
+  }
+  bool progress = false;
+  std::lock_guard<std::mutex> lk_api(e->api_mu);
+  while (picked.size() < e->cfg.group_trials) {
+    Trial t;
+    {
+      std::lock_guard<std::mutex> lk(e->q_mu);
+      if (e->queue.empty()) break;
+      t = e->queue.front();
+    }
+    hipError_t herr = hipSuccess;
+    char* mem = e->arena->alloc(t.lay.total, &herr);
+    if (!mem) {
+      if (picked.empty() && !others_in_flight) {
+        // Nothing else holds memory: this trial can never fit.
+        {
+          std::lock_guard<std::mutex> lk(e->q_mu);
+          e->queue.pop_front();
+          e->stats.queued -= 1;
+        }
+        fail(e, FDAS_ENOMEM, "request %llu needs %llu bytes of device state, beyond the HBM budget",
+             (unsigned long long)t.req.tag, (unsigned long long)t.lay.total);
+        fail_trial(e, t, FDAS_ENOMEM);
+        progress = true;
+        continue;
+      }
+      break;  // wait for memory to come back
+    }
+    {
+      std::lock_guard<std::mutex> lk(e->q_mu);
+      e->queue.pop_front();
+      e->stats.queued -= 1;
+      e->stats.in_flight += 1;
+    }
+    t.mem = mem;
+    picked.push_back(t);
+  }
+  if (picked.empty()) return progress;
+  // Pinned read-back area of the context.
+  uint64_t rb = 0;
+  for (const Trial& t : picked) rb += align_up(t.lay.readback_bytes);
+  if (rb > c.h_read_cap) {
+    (void)hipHostFree(c.h_read);
+    c.h_read = nullptr;
+    c.h_read_cap = 0;
+    if (host_alloc(e, &c.h_read, rb) == FDAS_OK) c.h_read_cap = rb;
+  }
+  fdas_status st = c.h_read ? FDAS_OK : FDAS_ENOMEM;
+  uint64_t off = 0;
+  std::vector<Trial*> ptrs;
+  for (Trial& t : picked) {
+    t.h_read = c.h_read + off;
+    off += align_up(t.lay.readback_bytes);
+    if (t.req.flags & FDAS_REQ_DEVICE_SPECTRUM) {
+      t.input = static_cast<const float2*>(t.req.spectrum);
+    } else {
+      t.input = reinterpret_cast<const float2*>(t.mem + t.lay.input);
+      if (st == FDAS_OK && hipMemcpyAsync(t.mem + t.lay.input, t.req.spectrum, 8ull * t.req.n_bins,
+                                          hipMemcpyHostToDevice, e->s_ftc) != hipSuccess)
+        st = fail(e, FDAS_EDEVICE, "spectrum upload of request %llu failed", (unsigned long long)t.req.tag);
+    }
+    ptrs.push_back(&t);
+  }
+  if (st == FDAS_OK) st = build_group(e, c.buf, ptrs);
+  if (st == FDAS_OK) st = upload_group(e, c.buf, group_ftc_jobs(c.buf), group_hsum_jobs(c.buf), e->s_ftc);
+  if (st == FDAS_OK) st = enqueue_group(e, c.buf, ptrs, nullptr, false);
+  if (st != FDAS_OK) {
+    // The streams may hold part of the group: drain them before the memory goes back.
+    (void)hipStreamSynchronize(e->s_ftc);
+    (void)hipStreamSynchronize(e->s_hsum);
+    for (Trial& t : picked) {
+      e->arena->free(t.mem);
+      {
+        std::lock_guard<std::mutex> lk(e->q_mu);
+        e->stats.in_flight -= 1;
+      }
+      fail_trial(e, t, st);
+    }
+    return true;
+  }
+  c.trials = std::move(picked);
+  c.busy = true;
+  c.seq = ++e->launch_seq;
+  std::lock_guard<std::mutex> lk(e->q_mu);
+  e->stats.groups_launched += 1;
+  e->stats.max_group = std::max<uint32_t>(e->stats.max_group, uint32_t(c.trials.size()));
+  return true;
+}
+
+void scheduler_loop(fdas_engine* e) {
+  (void)hipSetDevice(e->cfg.device);
+  for (;;) {
+    bool progress = false;
+    // Completions, oldest first.
+    for (;;) {
+      fdas_engine::Ctx* oldest = nullptr;
+      for (auto& c : e->ctxs)
+        if (c.busy && (!oldest || c.seq < oldest->seq)) oldest = &c;
+      if (!oldest) break;
+      if (hipEventQuery(oldest->buf.ev_done) != hipSuccess) {
+        (void)hipGetLastError();  // "not ready" must not reach the next launch's error check
+        break;
+      }
+      complete_ctx(e, *oldest);
+      progress = true;
+    }
+    bool any_busy = false, any_free = false;
+    for (auto& c : e->ctxs) (c.busy ? any_busy : any_free) = true;
+    if (any_free) {
+      for (auto& c : e->ctxs) {
+        if (c.busy) continue;
+        if (admit(e, c, any_busy)) {
+          progress = true;
+          any_busy = true;
+        }
+        break;
+      }
+    }
+    if (progress) continue;
+    std::unique_lock<std::mutex> lk(e->q_mu);
+    if (e->stop && e->queue.empty() && !any_busy) break;
+    if (!any_busy) {
+      e->q_cv.wait(lk, [&] { return e->stop || !e->queue.empty(); });
+    } else {
+      e->q_cv.wait_for(lk, std::chrono::microseconds(50));
+    }
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+void fdas_engine_default_config(fdas_engine_config* cfg) {
+  if (!cfg) return;
+  *cfg = fdas_engine_config{};
+  cfg->cands_per_harmonic = 4096;
+  cfg->spill_per_harmonic = 0;
+  cfg->group_trials = 8;
+  cfg->groups_in_flight = 3;
+  cfg->hbm_budget_bytes = 200ull * 1000 * 1000 * 1000;
+}
+
+static fdas_status engine_init(fdas_engine* e) {
+  const fdas_engine_config& c = e->cfg;
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || c.device < 0 || c.device >= n_dev) {
+    (void)hipGetLastError();
+    return fail(e, FDAS_EDEVICE, "device %d not available (%d devices)", c.device, n_dev);
+  }
+  EHIP(e, hipSetDevice(c.device));
+  int n_cu = 0;
+  EHIP(e, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
+  e->n_cu = uint32_t(n_cu);
+  EHIP(e, hipStreamCreateWithFlags(&e->s_ftc, hipStreamNonBlocking));
+  EHIP(e, hipStreamCreateWithFlags(&e->s_hsum, hipStreamNonBlocking));
+  for (uint32_t tile : kTileSizes) {
+    std::vector<float2> tw1, tw2;
+    make_twiddles(tile, tw1, tw2);
+    auto& tw = e->twiddles[tile];
+    fdas_status st;
+    if ((st = dev_alloc(e, &tw.first, tw1.size())) != FDAS_OK) return st;
+    if ((st = dev_alloc(e, &tw.second, tw2.size())) != FDAS_OK) return st;
+    EHIP(e, hipMemcpy(tw.first, tw1.data(), sizeof(float2) * tw1.size(), hipMemcpyHostToDevice));
+    EHIP(e, hipMemcpy(tw.second, tw2.data(), sizeof(float2) * tw2.size(), hipMemcpyHostToDevice));
+  }
+  e->arena = std::make_unique<Arena>(c.hbm_budget_bytes);
+  return FDAS_OK;
+}
+
+fdas_status fdas_engine_create(const fdas_engine_config* cfg_in, fdas_log_fn log, void* user, fdas_engine** out) {
+  if (!out) return FDAS_EINVAL;
+  *out = nullptr;
+  auto e = std::make_unique<fdas_engine>();
+  e->log = log;
+  e->user = user;
+  fdas_engine_default_config(&e->cfg);
+  if (cfg_in) e->cfg = *cfg_in;
+  fdas_engine_config& c = e->cfg;
+  if (c.group_trials == 0) c.group_trials = 8;
+  if (c.groups_in_flight == 0) c.groups_in_flight = 3;
+  if (c.hbm_budget_bytes == 0) c.hbm_budget_bytes = 200ull * 1000 * 1000 * 1000;
+  if (c.tile && c.tile != 1024 && c.tile != 2048 && c.tile != 4096)
+    return fail(e.get(), FDAS_EINVAL, "tile %u is not 1024, 2048 or 4096", c.tile);
+  if (c.rows_per_item > 256) return fail(e.get(), FDAS_EINVAL, "rows_per_item %u above 256", c.rows_per_item);
+  if (c.hsum_rows > 256) return fail(e.get(), FDAS_EINVAL, "hsum_rows %u above 256", c.hsum_rows);
+  if (c.ftc_workgroups_per_cu > 64)
+    return fail(e.get(), FDAS_EINVAL, "ftc_workgroups_per_cu %u above 64", c.ftc_workgroups_per_cu);
+  if (c.cands_per_harmonic < 1 || c.cands_per_harmonic > kSelectMaxCap)
+    return fail(e.get(), FDAS_EINVAL, "cands_per_harmonic %u outside 1 .. %u", c.cands_per_harmonic, kSelectMaxCap);
+  if (c.spill_per_harmonic && c.spill_per_harmonic < c.cands_per_harmonic)
+    return fail(e.get(), FDAS_EINVAL, "spill_per_harmonic %u below cands_per_harmonic %u", c.spill_per_harmonic,
+                c.cands_per_harmonic);
+  if (c.groups_in_flight < 2 || c.groups_in_flight > 8)
+    return fail(e.get(), FDAS_EINVAL, "groups_in_flight %u outside 2 .. 8", c.groups_in_flight);
+  if (c.group_trials > 256) return fail(e.get(), FDAS_EINVAL, "group_trials %u above 256", c.group_trials);
+  const fdas_status st = engine_init(e.get());
+  if (st != FDAS_OK) {
+    fdas_engine_destroy(e.release());
+    return st;
+  }
+  info(e.get(), "engine on device %d: HBM budget %.1f GB, groups of <= %u trials, %u groups in flight", c.device,
+       double(c.hbm_budget_bytes) / 1e9, c.group_trials, c.groups_in_flight);
+  *out = e.release();
+  return FDAS_OK;
+}
+
+void fdas_engine_destroy(fdas_engine* e) {
+  if (!e) return;
+  if (e->running) {
+    {
+      std::lock_guard<std::mutex> lk(e->q_mu);
+      e->stop = true;
+    }
+    e->q_cv.notify_all();
+    e->scheduler.join();
+  }
+  if (e->s_ftc) (void)hipSetDevice(e->cfg.device);
+  if (e->s_ftc) (void)hipStreamSynchronize(e->s_ftc);
+  if (e->s_hsum) (void)hipStreamSynchronize(e->s_hsum);
+  for (auto& c : e->ctxs) {
+    free_group(c.buf);
+    (void)hipHostFree(c.h_read);
+  }
+  for (auto& b : e->banks)
+    for (auto& kv : b->spectra) (void)hipFree(kv.second);
+  for (auto& kv : e->twiddles) {
+    (void)hipFree(kv.second.first);
+    (void)hipFree(kv.second.second);
+  }
+  (void)hipFree(e->zero_input);
+  e->arena.reset();
+  if (e->s_ftc) (void)hipStreamDestroy(e->s_ftc);
+  if (e->s_hsum) (void)hipStreamDestroy(e->s_hsum);
+  delete e;
+}
+
+const char* fdas_engine_last_error(const fdas_engine* e) {
+  if (!e) return "null engine";
+  thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(const_cast<fdas_engine*>(e)->err_mu);
+  copy = e->last_error;
+  return copy.c_str();
+}
+
+fdas_status fdas_engine_add_bank(fdas_engine* e, const float* taps_c64, uint32_t n_templates, uint32_t n_taps,
+                                 uint32_t* bank_id) {
+  if (!e) return FDAS_EINVAL;
+  if (!taps_c64 || !bank_id) return fail(e, FDAS_EINVAL, "null bank argument");
+  if (n_templates < 1 || n_templates > kMaxTemplates || n_templates % 2 == 0)
+    return fail(e, FDAS_EINVAL, "n_templates %u must be odd and <= %u", n_templates, kMaxTemplates);
+  if (n_taps < 1 || n_taps > 2048) return fail(e, FDAS_EINVAL, "n_taps %u outside 1 .. 2048", n_taps);
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  EHIP(e, hipSetDevice(e->cfg.device));
+  auto b = std::make_unique<Bank>();
+  b->n_templates = n_templates;
+  b->n_taps = n_taps;
+  b->taps.assign(taps_c64, taps_c64 + size_t(2) * n_templates * n_taps);
+  for (uint32_t tile : kTileSizes) {
+    if (e->cfg.tile && tile != e->cfg.tile) continue;
+    std::vector<uint32_t> extent;
+    const std::vector<std::complex<double>> spec =
+        taps_to_spectra(b->taps.data(), n_templates, n_taps, n_taps, tile, extent);
+    b->extent = extent;
+    b->max_extent = *std::max_element(extent.begin(), extent.end());
+    if (!tile_usable(tile, b->max_extent)) continue;
+    const std::vector<float2> perm = kernel_template_layout(spec, n_templates, tile);
+    float2* d = nullptr;
+    fdas_status st = dev_alloc(e, &d, perm.size());
+    if (st != FDAS_OK) return st;
+    b->spectra[tile] = d;
+    EHIP(e, hipMemcpy(d, perm.data(), sizeof(float2) * perm.size(), hipMemcpyHostToDevice));
+  }
+  if (b->spectra.empty()) {
+    const uint32_t need = 2 * overlap_for(b->max_extent);
+    return fail(e, FDAS_EUNSUPPORTED, "templates of %u taps need a tile of at least %u points%s", b->max_extent, need,
+                e->cfg.tile ? " (the configured tile is smaller)" : "");
+  }
+  *bank_id = uint32_t(e->banks.size());
+  info(e, "bank %u: %u templates x %u taps (longest %u), %zu tile sizes", *bank_id, n_templates, n_taps, b->max_extent,
+       b->spectra.size());
+  e->banks.push_back(std::move(b));
+  return FDAS_OK;
+}
+
+fdas_status fdas_engine_prepare(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part, uint32_t* tile) {
+  if (!e) return FDAS_EINVAL;
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  EHIP(e, hipSetDevice(e->cfg.device));
+  if (n_bins < 1 || n_bins > kMaxBins) return fail(e, FDAS_EINVAL, "n_bins %u outside 1 .. %u", n_bins, kMaxBins);
+  if (bank >= e->banks.size()) return fail(e, FDAS_EINVAL, "unknown bank %u", bank);
+  if (part > FDAS_ALL) return fail(e, FDAS_EINVAL, "invalid part %u", part);
+  if (e->zero_input_bins < n_bins) {
+    (void)hipFree(e->zero_input);
+    e->zero_input = nullptr;
+    e->zero_input_bins = 0;
+    fdas_status st = dev_alloc(e, &e->zero_input, kMaxBins);
+    if (st != FDAS_OK) return st;
+    EHIP(e, hipMemset(e->zero_input, 0, sizeof(float2) * kMaxBins));
+    e->zero_input_bins = kMaxBins;
+  }
+  const Shape* sh = nullptr;
+  const fdas_status st = get_shape(e, n_bins, bank, part, &sh);
+  if (st == FDAS_OK && tile) *tile = sh->tile;
+  return st;
+}
+
+// ---------------------------------------------------------------- plans
+
+fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t n, uint32_t flags, fdas_plan** out) {
+  if (!e) return FDAS_EINVAL;
+  if (!out || !reqs || n == 0) return fail(e, FDAS_EINVAL, "plan needs at least one request");
+  *out = nullptr;
+  // Shapes first (tuning takes the lock itself).
+  for (uint32_t i = 0; i < n; ++i) {
+    const fdas_status st = fdas_engine_prepare(e, reqs[i].n_bins, reqs[i].bank, reqs[i].part, nullptr);
+    if (st != FDAS_OK) return st;
+  }
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  EHIP(e, hipSetDevice(e->cfg.device));
+  auto p = std::make_unique<fdas_plan>();
+  p->eng = e;
+  p->use_graph = (flags & FDAS_PLAN_GRAPH) != 0;
+  p->trials.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    fdas_status st = make_trial(e, reqs[i], false, &p->trials[i]);
+    if (st != FDAS_OK) return st;
+  }
+  // Groups of consecutive trials.
+  const uint32_t G = e->cfg.group_trials, R = e->cfg.groups_in_flight;
+  for (uint32_t i = 0; i < n; i += G) {
+    std::vector<uint32_t> g;
+    for (uint32_t j = i; j < std::min(n, i + G); ++j) g.push_back(j);
+    p->groups.push_back(g);
+  }
+  const uint32_t n_groups = uint32_t(p->groups.size()), n_regions = std::min(R, n_groups);
+  std::vector<uint64_t> region_size(n_regions, 0);
+  for (uint32_t g = 0; g < n_groups; ++g) {
+    uint64_t s = 0;
+    for (uint32_t i : p->groups[g]) s += p->trials[i].lay.total;
+    region_size[g % n_regions] = std::max(region_size[g % n_regions], s);
+  }
+  auto cleanup = [&](fdas_status st) {
+    for (char* r : p->regions) e->arena->free(r);
+    for (char* m : p->inputs) e->arena->free(m);
+    for (auto& b : p->bufs) free_group(b);
+    (void)hipHostFree(p->h_read);
+    return st;
+  };
+  for (uint32_t r = 0; r < n_regions; ++r) {
+    hipError_t herr = hipSuccess;
+    char* m = e->arena->alloc(region_size[r], &herr);
+    if (!m)
+      return cleanup(fail(e, FDAS_ENOMEM, "plan needs %llu bytes of device state for %u groups in flight; the HBM "
+                                          "budget (%llu bytes, %llu reserved) cannot hold them",
+                          (unsigned long long)region_size[r], n_regions,
+                          (unsigned long long)e->cfg.hbm_budget_bytes, (unsigned long long)e->arena->reserved()));
+    p->regions.push_back(m);
+    p->region_bytes += region_size[r];
+  }
+  // Device copies of host spectra, one per trial, outside the reused regions.
+  p->inputs.assign(n, nullptr);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (p->trials[i].req.flags & FDAS_REQ_DEVICE_SPECTRUM) continue;
+    hipError_t herr = hipSuccess;
+    p->inputs[i] = e->arena->alloc(8ull * p->trials[i].req.n_bins, &herr);
+    if (!p->inputs[i])
+      return cleanup(fail(e, FDAS_ENOMEM, "no room in the HBM budget for the device copy of spectrum %u", i));
+    p->region_bytes += align_up(8ull * p->trials[i].req.n_bins);
+  }
+  uint64_t rb = 0;
+  for (const Trial& t : p->trials) rb += align_up(t.lay.readback_bytes);
+  fdas_status st = host_alloc(e, &p->h_read, rb);
+  if (st != FDAS_OK) return cleanup(st);
+  uint64_t roff = 0;
+  for (Trial& t : p->trials) {
+    t.h_read = p->h_read + roff;
+    roff += align_up(t.lay.readback_bytes);
+  }
+  p->bufs.resize(n_groups);
+  for (uint32_t g = 0; g < n_groups; ++g) {
+    uint64_t off = 0;
+    std::vector<Trial*> ptrs;
+    for (uint32_t i : p->groups[g]) {
+      Trial& t = p->trials[i];
+      t.mem = p->regions[g % n_regions] + off;
+      off += t.lay.total;
+      t.input = (t.req.flags & FDAS_REQ_DEVICE_SPECTRUM) ? static_cast<const float2*>(t.req.spectrum)
+                                                         : reinterpret_cast<const float2*>(p->inputs[i]);
+      ptrs.push_back(&t);
+    }
+    if ((st = build_group(e, p->bufs[g], ptrs)) != FDAS_OK) return cleanup(st);
+    if ((st = upload_group(e, p->bufs[g], group_ftc_jobs(p->bufs[g]), group_hsum_jobs(p->bufs[g]), e->s_ftc)) !=
+        FDAS_OK)
+      return cleanup(st);
+  }
+  if (hipStreamSynchronize(e->s_ftc) != hipSuccess) return cleanup(fail(e, FDAS_EDEVICE, "job upload failed"));
+  if (hipEventCreate(&p->ev_start) != hipSuccess || hipEventCreate(&p->ev_end) != hipSuccess)
+    return cleanup(fail(e, FDAS_EDEVICE, "event creation failed"));
+  info(e, "plan: %u trials in %u groups, %u groups resident (%.2f GB)", n, n_groups, n_regions,
+       double(p->region_bytes) / 1e9);
+  *out = p.release();
+  return FDAS_OK;
+}
+
+fdas_status fdas_plan_set_request(fdas_plan* p, uint32_t i, const void* spectrum, const float* thresholds) {
+  if (!p) return FDAS_EINVAL;
+  fdas_engine* e = p->eng;
+  if (p->launched) return fail(e, FDAS_ESTATE, "plan is running; wait for it before changing a request");
+  if (i >= p->trials.size()) return fail(e, FDAS_EINVAL, "request index %u outside the plan's %zu", i, p->trials.size());
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  EHIP(e, hipSetDevice(e->cfg.device));
+  Trial& t = p->trials[i];
+  if (spectrum) t.req.spectrum = spectrum;
+  if (thresholds)
+    for (uint32_t k = 0; k < t.req.n_harmonics; ++k) t.req.thresholds[k] = thresholds[k];
+  if (t.req.flags & FDAS_REQ_DEVICE_SPECTRUM) t.input = static_cast<const float2*>(t.req.spectrum);
+  // Rebuild the group's job arrays (pointers and thresholds live in device memory,
+  // so a captured graph needs no update).
+  for (uint32_t g = 0; g < p->groups.size(); ++g) {
+    if (std::find(p->groups[g].begin(), p->groups[g].end(), i) == p->groups[g].end()) continue;
+    std::vector<Trial*> ptrs;
+    for (uint32_t j : p->groups[g]) ptrs.push_back(&p->trials[j]);
+    fdas_status st = build_group(e, p->bufs[g], ptrs);
+    if (st != FDAS_OK) return st;
+    if ((st = upload_group(e, p->bufs[g], group_ftc_jobs(p->bufs[g]), group_hsum_jobs(p->bufs[g]), e->s_ftc)) !=
+        FDAS_OK)
+      return st;
+    EHIP(e, hipStreamSynchronize(e->s_ftc));
+  }
+  return FDAS_OK;
+}
+
+static fdas_status plan_enqueue(fdas_plan* p, bool timing) {
+  fdas_engine* e = p->eng;
+  const uint32_t R = uint32_t(p->regions.size());
+  for (uint32_t g = 0; g < p->groups.size(); ++g) {
+    std::vector<Trial*> ptrs;
+    for (uint32_t i : p->groups[g]) ptrs.push_back(&p->trials[i]);
+    const fdas_status st = enqueue_group(e, p->bufs[g], ptrs, g >= R ? p->bufs[g - R].ev_done : nullptr, timing);
+    if (st != FDAS_OK) return st;
+  }
+  // Join the summing stream back into the convolution stream.
+  EHIP(e, hipStreamWaitEvent(e->s_ftc, p->bufs.back().ev_done, 0));
+  return FDAS_OK;
+}
+
+fdas_status fdas_plan_launch(fdas_plan* p) {
+  if (!p) return FDAS_EINVAL;
+  fdas_engine* e = p->eng;
+  if (p->launched) return fail(e, FDAS_ESTATE, "plan already launched; wait for it first");
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  EHIP(e, hipSetDevice(e->cfg.device));
+  // Host spectra go to the plan's device copies first (outside any graph).
+  for (Trial& t : p->trials)
+    if (!(t.req.flags & FDAS_REQ_DEVICE_SPECTRUM))
+      EHIP(e, hipMemcpyAsync(const_cast<float2*>(t.input), t.req.spectrum, 8ull * t.req.n_bins, hipMemcpyHostToDevice,
+                             e->s_ftc));
+  EHIP(e, hipEventRecord(p->ev_start, e->s_ftc));
+  if (p->use_graph) {
+    if (!p->exec) {
+      EHIP(e, hipStreamBeginCapture(e->s_ftc, hipStreamCaptureModeThreadLocal));
+      fdas_status st = plan_enqueue(p, false);
+      hipGraph_t graph = nullptr;
+      const hipError_t end = hipStreamEndCapture(e->s_ftc, &graph);
+      if (st != FDAS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      EHIP(e, end);
+      p->graph = graph;
+      EHIP(e, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+      info(e, "plan captured as a HIP graph");
+    }
+    EHIP(e, hipGraphLaunch(p->exec, e->s_ftc));
+    p->eager_timed = false;
+  } else {
+    const fdas_status st = plan_enqueue(p, true);
+    if (st != FDAS_OK) return st;
+    p->eager_timed = true;
+  }
+  EHIP(e, hipEventRecord(p->ev_end, e->s_ftc));
+  p->launched = true;
+  p->n_launches += 1;
+  return FDAS_OK;
+}
+
+fdas_status fdas_plan_wait(fdas_plan* p, fdas_result* results) {
+  if (!p) return FDAS_EINVAL;
+  fdas_engine* e = p->eng;
+  if (!p->launched) return fail(e, FDAS_ESTATE, "plan not launched");
+  EHIP(e, hipSetDevice(e->cfg.device));
+  EHIP(e, hipEventSynchronize(p->ev_end));
+  p->launched = false;
+  fdas_status worst = FDAS_OK;
+  if (results) {
+    for (size_t i = 0; i < p->trials.size(); ++i) {
+      finish_result(e, p->trials[i], &results[i]);
+      results[i].latency_ms = 0.0;
+      if (results[i].status != FDAS_OK) worst = fdas_status(results[i].status);
+    }
+  }
+  if (worst == FDAS_EOVERFLOW)
+    return fail(e, FDAS_EOVERFLOW, "detections exceeded the spill capacity in at least one trial; its lists are "
+                                   "incomplete (raise the threshold or spill_per_harmonic)");
+  return FDAS_OK;
+}
+
+fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
+  if (!p || !out) return FDAS_EINVAL;
+  *out = fdas_plan_info{};
+  out->n_trials = uint32_t(p->trials.size());
+  out->n_groups = uint32_t(p->groups.size());
+  out->n_launches = p->n_launches;
+  out->graph = p->exec ? 1u : 0u;
+  out->region_bytes = p->region_bytes;
+  if (p->n_launches && !p->launched) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p->ev_start, p->ev_end) == hipSuccess) out->last_ms = ms;
+    (void)hipGetLastError();
+    if (p->eager_timed) {
+      for (const GroupBuf& g : p->bufs) {
+        float a = 0.f, b = 0.f;
+        if (hipEventElapsedTime(&a, g.ev_ftc0, g.ev_ftc1) == hipSuccess) out->ftc_ms += a;
+        if (hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
+        (void)hipGetLastError();
+      }
+    }
+  }
+  return FDAS_OK;
+}
+
+void fdas_plan_destroy(fdas_plan* p) {
+  if (!p) return;
+  fdas_engine* e = p->eng;
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  (void)hipSetDevice(e->cfg.device);
+  if (p->launched) (void)hipEventSynchronize(p->ev_end);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (auto& b : p->bufs) free_group(b);
+  for (char* r : p->regions) e->arena->free(r);
+  for (char* m : p->inputs) e->arena->free(m);
+  (void)hipHostFree(p->h_read);
+  if (p->ev_start) (void)hipEventDestroy(p->ev_start);
+  if (p->ev_end) (void)hipEventDestroy(p->ev_end);
+  delete p;
+}
+
+// ---------------------------------------------------------------- serving
+
+fdas_status fdas_engine_submit(fdas_engine* e, const fdas_request* req, fdas_result* result) {
+  if (!e) return FDAS_EINVAL;
+  if (!result) return fail(e, FDAS_EINVAL, "null result");
+  if (!req) return fail(e, FDAS_EINVAL, "null request");
+  Trial t;
+  {
+    fdas_status st = fdas_engine_prepare(e, req->n_bins, req->bank, req->part, nullptr);
+    if (st != FDAS_OK) return st;
+    std::lock_guard<std::mutex> lk(e->api_mu);
+    if ((st = make_trial(e, *req, true, &t)) != FDAS_OK) return st;
+  }
+  t.res = result;
+  t.t_submit = now_ms();
+  std::lock_guard<std::mutex> lk(e->q_mu);
+  if (e->stop) return fail(e, FDAS_ESTATE, "engine is shutting down");
+  start_scheduler(e);
+  e->queue.push_back(t);
+  e->stats.submitted += 1;
+  e->stats.queued += 1;
+  e->q_cv.notify_all();
+  return FDAS_OK;
+}
+
+fdas_status fdas_engine_poll(fdas_engine* e, fdas_result** done, uint32_t max, uint32_t* n_done, int32_t timeout_ms) {
+  if (!e || !n_done || (!done && max)) return FDAS_EINVAL;
+  *n_done = 0;
+  std::unique_lock<std::mutex> lk(e->q_mu);
+  if (timeout_ms < 0) {
+    e->done_cv.wait(lk, [&] { return !e->done.empty() || e->stats.completed >= e->stats.submitted; });
+  } else if (timeout_ms > 0) {
+    e->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !e->done.empty(); });
+  }
+  while (*n_done < max && !e->done.empty()) {
+    done[(*n_done)++] = e->done.front();
+    e->done.pop_front();
+  }
+  return FDAS_OK;
+}
+
+fdas_status fdas_engine_get_stats(fdas_engine* e, fdas_engine_stats* out) {
+  if (!e || !out) return FDAS_EINVAL;
+  fdas_engine_stats s;
+  {
+    std::lock_guard<std::mutex> lk(e->q_mu);
+    s = e->stats;
+  }
+  {
+    std::lock_guard<std::mutex> lk(e->api_mu);
+    s.arena_reserved_bytes = e->arena->reserved();
+    s.arena_in_use_bytes = e->arena->in_use();
+    s.arena_peak_bytes = e->arena->peak();
+  }
+  *out = s;
+  return FDAS_OK;
+}
+
+}  // extern "C"

@@ -50,6 +50,14 @@ inline Geometry make_geometry(uint32_t n_bins, uint32_t tile, uint32_t max_taps)
 // (hsum.py:109-114, hms_baseline.cl.mako:66-68).
 inline constexpr int sub_row(int r, int k, int c) { return c + (r - c) / k; }
 
+// Default spill capacity per plane: at least 262,144 entries and T*N/512, so
+// noise alone at the (1 + 6) x mean-power plane-1 threshold (an exp(-7)
+// ~ 1/1100 fraction of the plane) stays within it.
+inline constexpr uint32_t default_spill(uint32_t n_templates, uint32_t n_bins) {
+  const uint64_t by_size = (uint64_t(n_templates) * n_bins + 511) / 512;
+  return uint32_t(by_size > 262144u ? by_size : 262144u);
+}
+
 // 32-bit candidate location of the reference (gen_info.h:62-69, hsum.py:147).
 inline constexpr uint32_t encode_location(uint32_t harm, int tmpl, uint32_t freq, int c) {
   return (((harm - 1) & 0x7u) << 29) | ((uint32_t(tmpl + c) & 0x7fu) << 22) | (freq & 0x3fffffu);

@@ -36,8 +36,12 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
                                                         const float2* __restrict__ tw2) {
   using F = TileFFT<S>;
   constexpr int NT = F::NT;
+  // A one-wave workgroup (S = 1024) barriers for free: it keeps one LDS buffer
+  // (8.7 KB), so more workgroups fit beside it on a CU.  Wider tiles keep two
+  // buffers and save two barriers per inverse transform.
+  constexpr bool kSingle = (S == 1024);
   __shared__ float2 la[F::LDS_ELEMS];
-  __shared__ float2 lb[F::LDS_ELEMS];
+  __shared__ float2 lb[kSingle ? 1 : F::LDS_ELEMS];
   const int tid = threadIdx.x;
   F fft;
   fft.load_twiddles(tw1, tw2, tid);
@@ -81,7 +85,10 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
       float2 v[16];
 #pragma unroll
       for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], hn[s]);
-      fft.inverse(v, la, lb, tid);
+      if constexpr (kSingle)
+        fft.inverse_single(v, la, tid);
+      else
+        fft.inverse(v, la, lb, tid);
       {
         const float2* __restrict__ h = hbase + size_t(min(r + 1, r_end - 1)) * S + tid;  // unconditional: no branch
 #pragma unroll
@@ -166,15 +173,18 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 // plane and a wave-uniform fill count.  A full stage is flushed by the wave
 // alone (one global atomic for up to 64 entries); what is left at the end is
 // flushed with one atomic per plane per workgroup.
+template <int H>
 struct HsumStage {
-  uint32_t key[kHsumThreads / 64][kMaxHarmonics][kHsumWaveStage];
-  float pwr[kHsumThreads / 64][kMaxHarmonics][kHsumWaveStage];
-  uint32_t n[kHsumThreads / 64][kMaxHarmonics];
-  uint32_t base[kHsumThreads / 64][kMaxHarmonics];
+  uint32_t key[kHsumThreads / 64][H][kHsumWaveStage];
+  float pwr[kHsumThreads / 64][H][kHsumWaveStage];
+  uint32_t n[kHsumThreads / 64][H];
+  uint32_t base[kHsumThreads / 64][H];
 };
 
 // Writes the wave's n staged entries of plane p to the global spill list.
-__device__ __forceinline__ void wave_flush(HsumStage& st, const HsumJob& job, uint32_t wave, uint32_t p, uint32_t n) {
+template <int H>
+__device__ __forceinline__ void wave_flush(HsumStage<H>& st, const HsumJob& job, uint32_t wave, uint32_t p,
+                                           uint32_t n) {
   const uint32_t lane = lane_id();
   uint32_t g = 0;
   if (lane == 0) g = gatomic_add(job.counts + p, n);
@@ -237,7 +247,7 @@ __device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __res
 // the upper one: k - (f0 mod k) (f0 is a multiple of 4, so the 4 bins span at
 // most two sub-harmonic bins for every k >= 2).  Per thread, fixed for all rows.
 template <int H>
-__device__ __forceinline__ void hsum_row(HsumStage& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
+__device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
                                          uint32_t f0, uint32_t n_valid, const uint32_t (&split)[H],
                                          const float (&thr)[H], uint32_t (&wn)[H], uint32_t wave) {
   float hp[kHsumBinsPerThread];
@@ -283,7 +293,7 @@ __device__ __forceinline__ void hsum_row(HsumStage& st, const HsumJob& job, cons
 // row r is summed and thresholded.
 template <int H, bool VEC>
 __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __restrict__ jobs, uint32_t n_jobs) {
-  __shared__ HsumStage st;
+  __shared__ HsumStage<H> st;
   const HsumJob& job = jobs[find_job_hsum(jobs, n_jobs, blockIdx.x)];
   const int tid = threadIdx.x;
   const uint32_t wave = uint32_t(tid) >> 6;

@@ -14,8 +14,8 @@ namespace {
 constexpr double kPi = 3.14159265358979323846264338327950288;
 }
 
-uint32_t ftc_rows_per_item(uint32_t n_rows) {
-  const uint32_t target = 16;
+uint32_t ftc_rows_per_item(uint32_t n_rows, uint32_t target) {
+  target = std::max<uint32_t>(1, target);
   const uint32_t chunks = std::max<uint32_t>(1, (n_rows + target - 1) / target);
   return (n_rows + chunks - 1) / chunks;
 }
@@ -26,7 +26,7 @@ uint32_t ftc_rows_per_item(uint32_t n_rows) {
 // transforms) by dynamic programming over the row boundaries.
 std::vector<FtcJob> plan_ftc_rows(uint32_t tile, uint32_t n_bins, uint32_t max_taps,
                                   const std::vector<uint32_t>& row_taps, uint32_t row_begin, uint32_t n_rows,
-                                  uint32_t max_groups) {
+                                  uint32_t max_groups, uint32_t rows_target) {
   const uint32_t S = tile, N = n_bins;
   auto taps = [&](uint32_t r) { return row_taps.empty() ? max_taps : std::max(1u, std::min(row_taps[r], max_taps)); };
   auto tiles_for = [&](uint32_t o) { return (N + (S - o) - 1) / (S - o); };
@@ -64,7 +64,7 @@ std::vector<FtcJob> plan_ftc_rows(uint32_t tile, uint32_t n_bins, uint32_t max_t
     j.n_tiles = tiles_for(j.overlap);
     j.row_begin = row_begin + sg.first;
     j.n_rows = sg.second - sg.first;
-    j.rows_per_item = ftc_rows_per_item(j.n_rows);
+    j.rows_per_item = ftc_rows_per_item(j.n_rows, rows_target);
     j.n_chunks = (j.n_rows + j.rows_per_item - 1) / j.rows_per_item;
     jobs.push_back(j);
   }
@@ -119,8 +119,9 @@ void make_twiddles(uint32_t tile, std::vector<float2>& tw1, std::vector<float2>&
   }
 }
 
-void hsum_row_chunks(uint32_t n_rows, uint32_t* rows_per_block, uint32_t* n_row_chunks) {
-  *n_row_chunks = (n_rows + kHsumRowsPerBlock - 1) / kHsumRowsPerBlock;
+void hsum_row_chunks(uint32_t n_rows, uint32_t* rows_per_block, uint32_t* n_row_chunks, uint32_t target) {
+  target = std::max<uint32_t>(1, target);
+  *n_row_chunks = (n_rows + target - 1) / target;
   *rows_per_block = (n_rows + *n_row_chunks - 1) / *n_row_chunks;
 }
 

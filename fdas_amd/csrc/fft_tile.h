@@ -290,6 +290,34 @@ struct TileFFT {
     dft16<+1>(v);
   }
 
+  // As inverse(), with one LDS buffer for both exchanges (half the LDS, two more
+  // barriers).  Starts with a barrier, so the caller's previous readers of `lds`
+  // are done before it is rewritten.
+  __device__ __forceinline__ void inverse_single(float2 (&v)[16], float2* __restrict__ lds, int tid) const {
+    apply_dftD<+1>(v);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+#pragma unroll
+      for (int dd = 0; dd < D; ++dd) lds[dd * RS2 + tid + NT * i] = v[i * D + dd];
+    __syncthreads();
+    const int ka = tid & 15, d = tid >> 4;
+#pragma unroll
+    for (int kc = 0; kc < 16; ++kc) v[kc] = lds[d * RS2 + 16 * kc + ka];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w2[k]);
+    dft16<+1>(v);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lds[ka * RS1 + D * c + d] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[k * RS1 + tid];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w1[k]);
+    dft16<+1>(v);
+  }
+
  private:
   template <int DIR>
   __device__ __forceinline__ static void apply_dftD(float2 (&v)[16]) {

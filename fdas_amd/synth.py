@@ -123,3 +123,47 @@ def random_trial(seed: int, n_bins: int, rng_tones: int | None = None, z_max: fl
     tones = [Tone(float(r.uniform(1e4, hi)), float(r.uniform(-z_max, z_max)), 8, float(r.uniform(0.01, 0.03)))
              for _ in range(k)]
     return Workload(f"trial seed={seed}", n_bins, seed, tones)
+
+
+def _lshr(z, k):
+    """Logical right shift of int64 tensors holding uint64 bit patterns."""
+    return (z >> k) & ((1 << (64 - k)) - 1)
+
+
+def _as_i64(v: int) -> int:
+    v &= 0xFFFFFFFFFFFFFFFF
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def device_series(w: Workload, device="cuda"):
+    """``Workload.series`` computed with torch on ``device``: the same SplitMix64
+    stream (uint64 arithmetic on int64 bit patterns) and Box-Muller in FP64, so
+    the noise equals the host's up to the last bit of the FP64 transcendental
+    functions.  Bench and test input plumbing; the search never calls it."""
+    import torch
+    ns = 2 * w.n_bins
+    pairs = (ns + 1) // 2
+    i = torch.arange(1, 2 * pairs + 1, dtype=torch.int64, device=device)
+    z = _as_i64(w.seed) + i * _as_i64(int(_GOLDEN))
+    z = (z ^ _lshr(z, 30)) * _as_i64(int(_M1))
+    z = (z ^ _lshr(z, 27)) * _as_i64(int(_M2))
+    z = z ^ _lshr(z, 31)
+    u = _lshr(z, 11).to(torch.float64) * (1.0 / 9007199254740992.0)
+    u1, u2 = u[0::2], u[1::2]
+    r = torch.sqrt(-2.0 * torch.log1p(-u1))
+    th = 2.0 * np.pi * u2
+    s = torch.stack((r * torch.cos(th), r * torch.sin(th)), dim=1).reshape(-1)[:ns]
+    s = s.to(torch.float32).to(torch.float64)
+    n = torch.arange(ns, dtype=torch.float64, device=device)
+    for tn in w.tones:
+        phase = tn.f0 * n / ns + tn.z * n * n / (2.0 * ns * ns)
+        for h in range(1, tn.harmonics + 1):
+            s += tn.amplitude * torch.cos(2.0 * np.pi * h * phase)
+    return s.to(torch.float32)
+
+
+def device_spectrum(w: Workload, device="cuda"):
+    """``Workload.spectrum`` on ``device`` (complex64 tensor of N bins)."""
+    import torch
+    s = device_series(w, device).to(torch.float64)
+    return (torch.fft.rfft(s)[: w.n_bins] / (2 * w.n_bins)).to(torch.complex64).contiguous()

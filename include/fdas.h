@@ -70,7 +70,8 @@ typedef struct fdas_config {
   uint32_t tile;               /* FFT size S: 1024, 2048 or 4096 (2048) */
   uint32_t n_harmonics;        /* H planes, 1 .. 16 (8) */
   uint32_t cands_per_harmonic; /* candidates kept per plane, <= 8192 (512 -> 4096-entry lists) */
-  uint32_t spill_per_harmonic; /* detections buffered per plane before the cap (0: default 262144) */
+  uint32_t spill_per_harmonic; /* detections buffered per plane before the cap
+                                  (0: max(262144, T * N / 512)) */
   int32_t device;              /* HIP device ordinal */
   uint32_t flags;              /* FDAS_FLAG_* */
 } fdas_config;
@@ -205,6 +206,129 @@ fdas_status fdas_make_drift_bank(const double* drifts, uint32_t n, uint32_t max_
 /* Drifts of the reference bank generator (tmpl-gen.py:52-56, 127-133). */
 fdas_status fdas_reference_drifts(uint32_t n_templates, double max_accel, double unaccel_freq, double t_obs,
                                   double speed_of_light, double* out);
+
+/* ---------------------------------------------------------------------------
+ * Batched and served search (no counterpart in the reference, which runs one
+ * trial per launch on one buffer set; these replace its A/B double buffering
+ * for callers that search many dedispersion trials, FDAS.cpp:282-615).
+ *
+ * An engine holds template banks (any T <= 255, any tap count), the FFT
+ * twiddles of every tile size and a device-memory arena bounded by an HBM
+ * budget.  A trial is one spectrum with its own N, bank, harmonic count, FOP
+ * part and thresholds; its candidate lists are the same as those of the
+ * single-trial path (fdas_launch + fdas_retrieve_candidates_wide) run with the
+ * tile size the engine chose for that shape.
+ * ------------------------------------------------------------------------- */
+typedef struct fdas_engine fdas_engine;
+typedef struct fdas_plan fdas_plan;
+
+#define FDAS_ENGINE_MAX_HARMONICS 16
+
+typedef struct fdas_engine_config {
+  uint32_t tile;               /* FFT size for every shape; 0: autotuned per shape from 1024/2048/4096 */
+  uint32_t rows_per_item;      /* templates per FT-convolution workgroup; 0: autotuned (8/16/32) */
+  uint32_t hsum_rows;          /* rows per harmonic-summing workgroup; 0: autotuned (6/11/22) */
+  uint32_t cands_per_harmonic; /* candidates kept per plane, <= 8192 (default 4096) */
+  uint32_t spill_per_harmonic; /* detections buffered per plane before the cap
+                                  (0, the default: max(262144, T * N / 512) per trial) */
+  uint32_t group_trials;       /* most trials launched together (default 8) */
+  uint32_t groups_in_flight;   /* groups whose state is resident at once, 2..8 (default 3) */
+  int32_t device;
+  uint64_t hbm_budget_bytes;   /* device memory for per-trial state (default 200 GB) */
+  uint32_t ftc_workgroups_per_cu; /* FT-convolution workgroups resident per CU (a persistent grid), leaving
+                                     room for the harmonic summing that runs beside it; 0: one per item */
+  uint32_t pad_;
+} fdas_engine_config;
+
+/* Request flags. */
+#define FDAS_REQ_DEVICE_SPECTRUM 0x1u /* `spectrum` is device memory (else host memory) */
+
+typedef struct fdas_request {
+  const void* spectrum;        /* n_bins complex64 bins (re, im interleaved); must stay valid until the
+                                  trial completes */
+  uint32_t n_bins;             /* 1 .. 2^22 */
+  uint32_t bank;               /* id from fdas_engine_add_bank */
+  uint32_t n_harmonics;        /* 1 .. 16 */
+  uint32_t part;               /* fdas_part */
+  uint32_t flags;              /* FDAS_REQ_* */
+  uint32_t pad_;
+  float thresholds[FDAS_ENGINE_MAX_HARMONICS]; /* absolute threshold of plane k = 1..n_harmonics */
+  uint64_t tag;                /* copied to the result */
+} fdas_request;
+
+typedef struct fdas_result {
+  uint32_t* key;               /* caller-owned, n_harmonics * cands_per_harmonic entries: plane-major,
+                                  (row << 22) | bin, row = t + T/2, sorted ascending per plane */
+  float* power;                /* caller-owned, same layout: HP_k at that (row, bin) */
+  uint64_t tag;
+  int32_t status;              /* fdas_status of this trial (FDAS_EOVERFLOW: spill capacity exceeded) */
+  uint32_t n_harmonics;
+  uint32_t n_detected[FDAS_ENGINE_MAX_HARMONICS]; /* detections per plane before the cap */
+  uint32_t n_kept[FDAS_ENGINE_MAX_HARMONICS];     /* entries written per plane */
+  uint32_t tile;               /* FFT size the trial ran with */
+  uint32_t pad_;
+  double latency_ms;           /* served trials: submission to completion */
+} fdas_result;
+
+typedef struct fdas_engine_stats {
+  uint64_t submitted, completed, groups_launched;
+  uint64_t arena_reserved_bytes, arena_in_use_bytes, arena_peak_bytes;
+  uint32_t queued, in_flight;  /* served trials waiting for admission / on the device */
+  uint32_t max_group;          /* largest group launched */
+  uint32_t pad_;
+} fdas_engine_stats;
+
+typedef struct fdas_plan_info {
+  uint32_t n_trials, n_groups, n_launches, graph;
+  uint64_t region_bytes;       /* device bytes of the plan's per-trial state */
+  float last_ms;               /* device time of the last execution (start to end events) */
+  float ftc_ms, hsum_ms;       /* summed FT-convolution / harmonic-summing kernel spans of the last eager
+                                  execution (0 under a graph) */
+} fdas_plan_info;
+
+#define FDAS_PLAN_GRAPH 0x1u   /* capture the plan's launches once as a HIP graph and replay it */
+
+void fdas_engine_default_config(fdas_engine_config* cfg);
+fdas_status fdas_engine_create(const fdas_engine_config* cfg, fdas_log_fn log, void* user, fdas_engine** out);
+/* Completes every submitted trial, then releases all device state. */
+void fdas_engine_destroy(fdas_engine* eng);
+const char* fdas_engine_last_error(const fdas_engine* eng);
+
+/* Registers a bank of n_templates (odd) x n_taps complex64 taps (the reference's
+ * fdas_templates_*.npy layout, tmpl-gen.py:154-159); their spectra are formed
+ * for every usable tile size and stay resident. */
+fdas_status fdas_engine_add_bank(fdas_engine* eng, const float* taps_c64, uint32_t n_templates, uint32_t n_taps,
+                                 uint32_t* bank_id);
+
+/* Fixes (autotuning if configured) the tile size and work shapes of
+ * (n_bins, bank, part) ahead of traffic; *tile receives the tile size (may be NULL). */
+fdas_status fdas_engine_prepare(fdas_engine* eng, uint32_t n_bins, uint32_t bank, uint32_t part, uint32_t* tile);
+
+/* Batch: a fixed list of trials, split into groups of at most group_trials; the
+ * FT convolution of one group runs on one stream while harmonic summing and
+ * selection of the previous group run on another. */
+fdas_status fdas_plan_create(fdas_engine* eng, const fdas_request* reqs, uint32_t n, uint32_t flags,
+                             fdas_plan** out);
+/* Replaces trial i's spectrum pointer and thresholds (same shape) before the next launch. */
+fdas_status fdas_plan_set_request(fdas_plan* plan, uint32_t i, const void* spectrum, const float* thresholds);
+fdas_status fdas_plan_launch(fdas_plan* plan);
+/* Waits for the launch and fills results[0..n) (NULL: only waits). */
+fdas_status fdas_plan_wait(fdas_plan* plan, fdas_result* results);
+fdas_status fdas_plan_get_info(const fdas_plan* plan, fdas_plan_info* info);
+void fdas_plan_destroy(fdas_plan* plan);
+
+/* Serving: trials are queued, admitted while the HBM budget and the groups in
+ * flight allow, and launched in groups formed from whatever is waiting, so
+ * requests that arrive while others run join the next group.  `result` (and
+ * its key/power arrays and the request's spectrum) must stay valid until the
+ * result is returned by fdas_engine_poll.  Calls are thread-safe. */
+fdas_status fdas_engine_submit(fdas_engine* eng, const fdas_request* req, fdas_result* result);
+/* Returns up to `max` completed results (their addresses as passed to submit),
+ * waiting up to timeout_ms for the first (0: no wait; < 0: until one completes
+ * or nothing is pending). */
+fdas_status fdas_engine_poll(fdas_engine* eng, fdas_result** done, uint32_t max, uint32_t* n_done,
+                             int32_t timeout_ms);
+fdas_status fdas_engine_get_stats(fdas_engine* eng, fdas_engine_stats* stats);
 
 #ifdef __cplusplus
 }

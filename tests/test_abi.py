@@ -90,3 +90,39 @@ def test_drift_bank_matches_scipy_fresnel(n_templates):
 def test_drift_bank_rejects_too_long_template():
     with pytest.raises(ValueError):
         fdas_amd.make_drift_bank(np.array([300.0]), 421)  # 600 taps > 421
+
+
+@pytest.mark.parametrize("field,value", [("tile", 3000), ("cands_per_harmonic", 0), ("cands_per_harmonic", 8193),
+                                         ("groups_in_flight", 1), ("groups_in_flight", 9), ("group_trials", 257),
+                                         ("spill_per_harmonic", 100), ("rows_per_item", 1000)])
+def test_invalid_engine_config_rejected_before_device(field, value):
+    lib = N.load()
+    cfg = N.EngineConfig()
+    lib.fdas_engine_default_config(C.byref(cfg))
+    assert (cfg.cands_per_harmonic, cfg.group_trials, cfg.groups_in_flight) == (4096, 8, 3)
+    assert cfg.hbm_budget_bytes == 200 * 10**9
+    setattr(cfg, field, value)
+    msgs = []
+    cb = N.LOG_FN(lambda lvl, m, u: msgs.append((lvl, m.decode())))
+    eng = C.c_void_p()
+    assert lib.fdas_engine_create(C.byref(cfg), cb, None, C.byref(eng)) == 1
+    assert not eng
+    assert msgs and msgs[-1][0] == 2 and msgs[-1][1].startswith("[ERROR]")
+
+
+def test_engine_null_arguments():
+    lib = N.load()
+    assert lib.fdas_engine_create(None, N.LOG_FN(0), None, None) == 1
+    lib.fdas_engine_destroy(None)
+    lib.fdas_plan_destroy(None)
+    assert lib.fdas_engine_last_error(None) == b"null engine"
+    assert lib.fdas_plan_launch(None) == 1
+    assert lib.fdas_engine_submit(None, None, None) == 1
+
+
+def test_engine_structures_match_header():
+    """ctypes layouts of the engine structures agree with the C sizes the header implies."""
+    assert C.sizeof(N.EngineConfig) == 48
+    assert C.sizeof(N.Request) == 8 + 6 * 4 + 16 * 4 + 8
+    assert C.sizeof(N.Result) == 8 + 8 + 8 + 4 + 4 + 64 + 64 + 4 + 4 + 8
+    assert C.sizeof(N.PlanInfo) == 16 + 8 + 12 + 4
