@@ -153,14 +153,18 @@ struct Shape {
   uint32_t n_items = 0;
 };
 
-// Byte offsets of one trial's device state inside its arena block.  The first
-// four arrays are contiguous so one copy reads the lists back; out_n .. hist is
-// zeroed per run.
+// Byte offsets of one trial's device state, in two parts.  The small part
+// (lists, counters, selection histogram) is zeroed before every run and read
+// back; the large part (selection pool, spill lists, input copy, FOP) is not.
+// Plans keep the small parts of all their trials in one array, so a group
+// zeroes and reads back its trials with one launch and one copy.
 struct TrialLayout {
-  uint64_t out_n, counts, out_key, out_pwr, compact_n, hist, compact_key, compact_pwr, spill_key, spill_pwr, input,
-      fop, total;
-  uint64_t readback_bytes;
-  uint32_t zero_words;
+  // small part, from its base
+  uint64_t out_n, counts, out_key, out_pwr, compact_n, hist, small_bytes;
+  uint64_t readback_bytes;  // out_n .. out_pwr
+  // large part, from its base
+  uint64_t compact_key, compact_pwr, spill_key, spill_pwr, input, fop, large_bytes;
+  uint64_t total;  // small_bytes + large_bytes
 };
 
 TrialLayout trial_layout(uint32_t n_bins, uint32_t T, uint32_t H, uint32_t cap, uint32_t spill, bool own_input) {
@@ -179,9 +183,9 @@ TrialLayout trial_layout(uint32_t n_bins, uint32_t T, uint32_t H, uint32_t cap, 
   o += 4ull * H;
   l.hist = o;
   o += 4ull * H * kSelectBins;
-  l.zero_words = uint32_t(o / 4);
+  l.small_bytes = align_up(o);
   const uint64_t ccap = uint64_t(cap) + kSelectPool;
-  o = align_up(o);
+  o = 0;
   l.compact_key = o;
   o = align_up(o + 4ull * H * ccap);
   l.compact_pwr = o;
@@ -194,7 +198,8 @@ TrialLayout trial_layout(uint32_t n_bins, uint32_t T, uint32_t H, uint32_t cap, 
   if (own_input) o = align_up(o + 8ull * n_bins);
   l.fop = o;
   o = align_up(o + 4ull * T * n_bins);
-  l.total = o;
+  l.large_bytes = o;
+  l.total = l.small_bytes + l.large_bytes;
   return l;
 }
 
@@ -206,9 +211,10 @@ struct Trial {
   const Bank* bank = nullptr;
   TrialLayout lay{};
   uint32_t spill = 0;             // spill capacity per plane
-  char* mem = nullptr;            // device block (arena)
+  char* small = nullptr;          // device small part (served trials: also the arena block)
+  char* mem = nullptr;            // device large part
   const float2* input = nullptr;  // device spectrum
-  char* h_read = nullptr;         // pinned read-back area (lay.readback_bytes)
+  char* h_read = nullptr;         // pinned mirror of the small part
   double t_submit = 0.0;
 };
 
@@ -223,7 +229,10 @@ struct GroupBuf {
   SelectJob* d_sel = nullptr;
   SelectJob* h_sel = nullptr;
   uint32_t sel_cap = 0;
+  // ev_ftc1 / ev_done order the streams; ev_ftc0 / ev_ftc_t1 / ev_hsum0 / ev_hsum1 only time the kernels.
   hipEvent_t ev_ftc0 = nullptr, ev_ftc1 = nullptr, ev_hsum0 = nullptr, ev_hsum1 = nullptr, ev_done = nullptr;
+  hipEvent_t ev_ftc_t1 = nullptr;
+  hipEvent_t ev_hsum_done = nullptr;  // harmonic summing finished (the next group's convolution waits on it)
   struct FtcLaunch {
     uint32_t tile, first, n_jobs, n_items;
   };
@@ -234,6 +243,10 @@ struct GroupBuf {
   std::vector<FtcLaunch> ftc;
   std::vector<HsumLaunch> hsum;
   uint32_t n_sel = 0;
+  // Small parts of the group's trials when they are one contiguous span (plans).
+  char* small_span = nullptr;
+  char* h_span = nullptr;
+  uint64_t span_bytes = 0;
 };
 
 }  // namespace
@@ -271,6 +284,7 @@ struct fdas_engine {
   };
   std::vector<Ctx> ctxs;
   uint64_t launch_seq = 0;
+  hipEvent_t last_hsum = nullptr;  // summing-done event of the last served group
   fdas_engine_stats stats{};
 };
 
@@ -281,7 +295,8 @@ struct fdas_plan {
   std::vector<GroupBuf> bufs;
   std::vector<char*> regions;
   std::vector<char*> inputs;  // plan-owned device copies of host spectra (nullptr for device spectra)
-  char* h_read = nullptr;
+  char* small_all = nullptr;  // small parts of all trials, in trial order
+  char* h_read = nullptr;     // its pinned mirror
   uint64_t region_bytes = 0;
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
   hipGraph_t graph = nullptr;
@@ -380,14 +395,15 @@ void free_group(GroupBuf& g) {
   (void)hipHostFree(g.h_ftc);
   (void)hipHostFree(g.h_hsum);
   (void)hipHostFree(g.h_sel);
-  for (hipEvent_t ev : {g.ev_ftc0, g.ev_ftc1, g.ev_hsum0, g.ev_hsum1, g.ev_done})
+  for (hipEvent_t ev : {g.ev_ftc0, g.ev_ftc1, g.ev_hsum0, g.ev_hsum1, g.ev_done, g.ev_ftc_t1, g.ev_hsum_done})
     if (ev) (void)hipEventDestroy(ev);
   g = GroupBuf{};
 }
 
 fdas_status group_events(fdas_engine* e, GroupBuf& g) {
-  for (hipEvent_t* ev : {&g.ev_ftc0, &g.ev_ftc1, &g.ev_hsum0, &g.ev_hsum1, &g.ev_done})
+  for (hipEvent_t* ev : {&g.ev_ftc0, &g.ev_ftc1, &g.ev_hsum0, &g.ev_hsum1, &g.ev_done, &g.ev_ftc_t1})
     if (!*ev) EHIP(e, hipEventCreate(ev));
+  if (!g.ev_hsum_done) EHIP(e, hipEventCreateWithFlags(&g.ev_hsum_done, hipEventDisableTiming));
   return FDAS_OK;
 }
 
@@ -462,6 +478,7 @@ fdas_status build_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& 
     for (const Trial* t : trials) {
       if (t->req.n_harmonics != H) continue;
       char* m = t->mem;
+      char* sm = t->small;
       const TrialLayout& l = t->lay;
       const uint32_t spill = t->spill;
       HsumJob& j = g.h_hsum[jh++];
@@ -477,7 +494,7 @@ fdas_status build_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& 
       hsum_row_chunks(j.n_rows, &j.rows_per_block, &j.n_row_chunks, t->shape->hsum_rows);
       j.spill_key = reinterpret_cast<uint32_t*>(m + l.spill_key);
       j.spill_pwr = reinterpret_cast<float*>(m + l.spill_pwr);
-      j.counts = reinterpret_cast<uint32_t*>(m + l.counts);
+      j.counts = reinterpret_cast<uint32_t*>(sm + l.counts);
       for (uint32_t k = 0; k < H; ++k) j.thr[k] = t->req.thresholds[k];
       L.n_blocks += (j.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
       L.vec = L.vec && (j.n_bins & 3u) == 0;
@@ -490,13 +507,13 @@ fdas_status build_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& 
         q.cap = cap;
         q.thr = t->req.thresholds[p];
         q.compact_cap = ccap;
-        q.hist = reinterpret_cast<uint32_t*>(m + l.hist) + size_t(p) * kSelectBins;
+        q.hist = reinterpret_cast<uint32_t*>(sm + l.hist) + size_t(p) * kSelectBins;
         q.compact_key = reinterpret_cast<uint32_t*>(m + l.compact_key) + size_t(p) * ccap;
         q.compact_pwr = reinterpret_cast<float*>(m + l.compact_pwr) + size_t(p) * ccap;
-        q.compact_n = reinterpret_cast<uint32_t*>(m + l.compact_n) + p;
-        q.out_key = reinterpret_cast<uint32_t*>(m + l.out_key) + size_t(p) * cap;
-        q.out_pwr = reinterpret_cast<float*>(m + l.out_pwr) + size_t(p) * cap;
-        q.out_n = reinterpret_cast<uint32_t*>(m + l.out_n) + p;
+        q.compact_n = reinterpret_cast<uint32_t*>(sm + l.compact_n) + p;
+        q.out_key = reinterpret_cast<uint32_t*>(sm + l.out_key) + size_t(p) * cap;
+        q.out_pwr = reinterpret_cast<float*>(sm + l.out_pwr) + size_t(p) * cap;
+        q.out_n = reinterpret_cast<uint32_t*>(sm + l.out_n) + p;
       }
     }
     L.n_jobs = jh - L.first;
@@ -504,6 +521,19 @@ fdas_status build_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& 
   }
   g.n_sel = js;
   (void)n_sel;
+  // One zeroing launch and one read-back copy when the small parts are contiguous.
+  g.small_span = nullptr;
+  g.h_span = nullptr;
+  g.span_bytes = 0;
+  bool contiguous = !trials.empty();
+  for (size_t i = 1; i < trials.size() && contiguous; ++i)
+    contiguous = trials[i]->small == trials[i - 1]->small + trials[i - 1]->lay.small_bytes &&
+                 trials[i]->h_read == trials[i - 1]->h_read + trials[i - 1]->lay.small_bytes;
+  if (contiguous) {
+    g.small_span = trials.front()->small;
+    g.h_span = trials.front()->h_read;
+    g.span_bytes = uint64_t(trials.back()->small - trials.front()->small) + trials.back()->lay.small_bytes;
+  }
   return FDAS_OK;
 }
 
@@ -528,27 +558,44 @@ uint32_t group_hsum_jobs(const GroupBuf& g) {
 
 // Enqueues one group: convolution on s_ftc, then (after its end event) zeroing,
 // summing, selection and read-back on s_hsum.  wait_before: an event s_ftc waits
-// on first (the group whose memory this one reuses), or nullptr.
+// on first (the group whose memory this one reuses), or nullptr.  prev_hsum: the
+// previous group's summing-done event, or nullptr: the convolution starts after
+// it, since the two kernels only slow each other down when they share the CUs
+// (both VALU-heavy), while the selection and read-back that follow the summing
+// still run beside the next convolution.  Timing events bracket the two kernels
+// (not under graph capture, where this runtime rejects externally visible
+// event records).
 fdas_status enqueue_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& trials, hipEvent_t wait_before,
-                          bool timing_events) {
+                          hipEvent_t prev_hsum, bool timing_events) {
   hipStream_t A = e->s_ftc, B = e->s_hsum;
   if (wait_before) EHIP(e, hipStreamWaitEvent(A, wait_before, 0));
+  if (prev_hsum) EHIP(e, hipStreamWaitEvent(A, prev_hsum, 0));
   if (timing_events) EHIP(e, hipEventRecord(g.ev_ftc0, A));
   for (const auto& L : g.ftc) {
     const auto& tw = e->twiddles.at(L.tile);
     EHIP(e, launch_ftconv(int(L.tile), g.d_ftc + L.first, L.n_jobs, L.n_items, tw.first, tw.second,
                           ftc_grid(e, L.n_items), A));
   }
+  if (timing_events) EHIP(e, hipEventRecord(g.ev_ftc_t1, A));
   EHIP(e, hipEventRecord(g.ev_ftc1, A));
   EHIP(e, hipStreamWaitEvent(B, g.ev_ftc1, 0));
-  for (const Trial* t : trials)
-    EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(t->mem + t->lay.out_n), t->lay.zero_words, B));
+  if (g.small_span) {
+    EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(g.small_span), uint32_t(g.span_bytes / 4), B));
+  } else {
+    for (const Trial* t : trials)
+      EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(t->small), uint32_t(t->lay.small_bytes / 4), B));
+  }
   if (timing_events) EHIP(e, hipEventRecord(g.ev_hsum0, B));
   for (const auto& L : g.hsum) EHIP(e, launch_hsum(g.d_hsum + L.first, L.n_jobs, L.n_blocks, L.H, L.vec, B));
   if (timing_events) EHIP(e, hipEventRecord(g.ev_hsum1, B));
+  EHIP(e, hipEventRecord(g.ev_hsum_done, B));
   EHIP(e, launch_select(g.d_sel, g.n_sel, B));
-  for (const Trial* t : trials)
-    EHIP(e, hipMemcpyAsync(t->h_read, t->mem + t->lay.out_n, t->lay.readback_bytes, hipMemcpyDeviceToHost, B));
+  if (g.small_span) {
+    EHIP(e, hipMemcpyAsync(g.h_span, g.small_span, g.span_bytes, hipMemcpyDeviceToHost, B));
+  } else {
+    for (const Trial* t : trials)
+      EHIP(e, hipMemcpyAsync(t->h_read, t->small, t->lay.readback_bytes, hipMemcpyDeviceToHost, B));
+  }
   EHIP(e, hipEventRecord(g.ev_done, B));
   return FDAS_OK;
 }
@@ -656,8 +703,10 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
     const uint32_t spill = e->cfg.spill_per_harmonic ? e->cfg.spill_per_harmonic : default_spill(T, n_bins);
     const TrialLayout lay = trial_layout(n_bins, T, 8, e->cfg.cands_per_harmonic, spill, false);
     hipError_t herr = hipSuccess;
-    char* mem = e->arena->alloc(lay.total, &herr);
-    if (!mem) {
+    char* block = e->arena->alloc(lay.total, &herr);
+    char* small = block;
+    char* mem = block ? block + lay.small_bytes : nullptr;
+    if (!block) {
       info(e, "autotune of N=%u bank %u skipped: no room in the HBM budget; using tile %u", n_bins, bank_id,
            out->tile);
     } else {
@@ -709,7 +758,7 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
         j.spill_cap = spill;
         j.spill_key = reinterpret_cast<uint32_t*>(mem + lay.spill_key);
         j.spill_pwr = reinterpret_cast<float*>(mem + lay.spill_pwr);
-        j.counts = reinterpret_cast<uint32_t*>(mem + lay.counts);
+        j.counts = reinterpret_cast<uint32_t*>(small + lay.counts);
         for (uint32_t k = 0; k < kMaxHarmonics; ++k) j.thr[k] = FLT_MAX;
         if (!tune_ftc) {
           // The FOP block is uninitialised: clear it so the thresholds hold.
@@ -743,7 +792,7 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
         }
       }
       free_group(g);
-      e->arena->free(mem);
+      e->arena->free(block);
       if (st != FDAS_OK) return st;
     }
   }
@@ -805,7 +854,7 @@ void complete_ctx(fdas_engine* e, fdas_engine::Ctx& c) {
     for (Trial& tr : c.trials) {
       finish_result(e, tr, tr.res);
       tr.res->latency_ms = t - tr.t_submit;
-      e->arena->free(tr.mem);
+      e->arena->free(tr.small);
       out.push_back(tr.res);
     }
   }
@@ -872,13 +921,14 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight) {
       e->stats.queued -= 1;
       e->stats.in_flight += 1;
     }
-    t.mem = mem;
+    t.small = mem;
+    t.mem = mem + t.lay.small_bytes;
     picked.push_back(t);
   }
   if (picked.empty()) return progress;
   // Pinned read-back area of the context.
   uint64_t rb = 0;
-  for (const Trial& t : picked) rb += align_up(t.lay.readback_bytes);
+  for (const Trial& t : picked) rb += t.lay.small_bytes;
   if (rb > c.h_read_cap) {
     (void)hipHostFree(c.h_read);
     c.h_read = nullptr;
@@ -890,7 +940,7 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight) {
   std::vector<Trial*> ptrs;
   for (Trial& t : picked) {
     t.h_read = c.h_read + off;
-    off += align_up(t.lay.readback_bytes);
+    off += t.lay.small_bytes;
     if (t.req.flags & FDAS_REQ_DEVICE_SPECTRUM) {
       t.input = static_cast<const float2*>(t.req.spectrum);
     } else {
@@ -903,13 +953,14 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight) {
   }
   if (st == FDAS_OK) st = build_group(e, c.buf, ptrs);
   if (st == FDAS_OK) st = upload_group(e, c.buf, group_ftc_jobs(c.buf), group_hsum_jobs(c.buf), e->s_ftc);
-  if (st == FDAS_OK) st = enqueue_group(e, c.buf, ptrs, nullptr, false);
+  if (st == FDAS_OK) st = enqueue_group(e, c.buf, ptrs, nullptr, e->last_hsum, false);
+  if (st == FDAS_OK) e->last_hsum = c.buf.ev_hsum_done;
   if (st != FDAS_OK) {
     // The streams may hold part of the group: drain them before the memory goes back.
     (void)hipStreamSynchronize(e->s_ftc);
     (void)hipStreamSynchronize(e->s_hsum);
     for (Trial& t : picked) {
-      e->arena->free(t.mem);
+      e->arena->free(t.small);
       {
         std::lock_guard<std::mutex> lk(e->q_mu);
         e->stats.in_flight -= 1;
@@ -1179,10 +1230,11 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
   std::vector<uint64_t> region_size(n_regions, 0);
   for (uint32_t g = 0; g < n_groups; ++g) {
     uint64_t s = 0;
-    for (uint32_t i : p->groups[g]) s += p->trials[i].lay.total;
+    for (uint32_t i : p->groups[g]) s += p->trials[i].lay.large_bytes;
     region_size[g % n_regions] = std::max(region_size[g % n_regions], s);
   }
   auto cleanup = [&](fdas_status st) {
+    if (p->small_all) e->arena->free(p->small_all);
     for (char* r : p->regions) e->arena->free(r);
     for (char* m : p->inputs) e->arena->free(m);
     for (auto& b : p->bufs) free_group(b);
@@ -1210,14 +1262,22 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
       return cleanup(fail(e, FDAS_ENOMEM, "no room in the HBM budget for the device copy of spectrum %u", i));
     p->region_bytes += align_up(8ull * p->trials[i].req.n_bins);
   }
+  // Small parts of all trials in one array (and its pinned mirror), in trial order.
   uint64_t rb = 0;
-  for (const Trial& t : p->trials) rb += align_up(t.lay.readback_bytes);
+  for (const Trial& t : p->trials) rb += t.lay.small_bytes;
+  {
+    hipError_t herr = hipSuccess;
+    p->small_all = e->arena->alloc(rb, &herr);
+    if (!p->small_all) return cleanup(fail(e, FDAS_ENOMEM, "no room in the HBM budget for the plan's lists"));
+    p->region_bytes += align_up(rb);
+  }
   fdas_status st = host_alloc(e, &p->h_read, rb);
   if (st != FDAS_OK) return cleanup(st);
   uint64_t roff = 0;
   for (Trial& t : p->trials) {
+    t.small = p->small_all + roff;
     t.h_read = p->h_read + roff;
-    roff += align_up(t.lay.readback_bytes);
+    roff += t.lay.small_bytes;
   }
   p->bufs.resize(n_groups);
   for (uint32_t g = 0; g < n_groups; ++g) {
@@ -1226,7 +1286,7 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
     for (uint32_t i : p->groups[g]) {
       Trial& t = p->trials[i];
       t.mem = p->regions[g % n_regions] + off;
-      off += t.lay.total;
+      off += t.lay.large_bytes;
       t.input = (t.req.flags & FDAS_REQ_DEVICE_SPECTRUM) ? static_cast<const float2*>(t.req.spectrum)
                                                          : reinterpret_cast<const float2*>(p->inputs[i]);
       ptrs.push_back(&t);
@@ -1273,13 +1333,14 @@ fdas_status fdas_plan_set_request(fdas_plan* p, uint32_t i, const void* spectrum
   return FDAS_OK;
 }
 
-static fdas_status plan_enqueue(fdas_plan* p, bool timing) {
+static fdas_status plan_enqueue(fdas_plan* p, bool capturing) {
   fdas_engine* e = p->eng;
   const uint32_t R = uint32_t(p->regions.size());
   for (uint32_t g = 0; g < p->groups.size(); ++g) {
     std::vector<Trial*> ptrs;
     for (uint32_t i : p->groups[g]) ptrs.push_back(&p->trials[i]);
-    const fdas_status st = enqueue_group(e, p->bufs[g], ptrs, g >= R ? p->bufs[g - R].ev_done : nullptr, timing);
+    const fdas_status st = enqueue_group(e, p->bufs[g], ptrs, g >= R ? p->bufs[g - R].ev_done : nullptr,
+                                         g > 0 ? p->bufs[g - 1].ev_hsum_done : nullptr, !capturing);
     if (st != FDAS_OK) return st;
   }
   // Join the summing stream back into the convolution stream.
@@ -1302,7 +1363,7 @@ fdas_status fdas_plan_launch(fdas_plan* p) {
   if (p->use_graph) {
     if (!p->exec) {
       EHIP(e, hipStreamBeginCapture(e->s_ftc, hipStreamCaptureModeThreadLocal));
-      fdas_status st = plan_enqueue(p, false);
+      fdas_status st = plan_enqueue(p, true);
       hipGraph_t graph = nullptr;
       const hipError_t end = hipStreamEndCapture(e->s_ftc, &graph);
       if (st != FDAS_OK) {
@@ -1317,7 +1378,7 @@ fdas_status fdas_plan_launch(fdas_plan* p) {
     EHIP(e, hipGraphLaunch(p->exec, e->s_ftc));
     p->eager_timed = false;
   } else {
-    const fdas_status st = plan_enqueue(p, true);
+    const fdas_status st = plan_enqueue(p, false);
     if (st != FDAS_OK) return st;
     p->eager_timed = true;
   }
@@ -1360,13 +1421,12 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p->ev_start, p->ev_end) == hipSuccess) out->last_ms = ms;
     (void)hipGetLastError();
-    if (p->eager_timed) {
-      for (const GroupBuf& g : p->bufs) {
-        float a = 0.f, b = 0.f;
-        if (hipEventElapsedTime(&a, g.ev_ftc0, g.ev_ftc1) == hipSuccess) out->ftc_ms += a;
-        if (hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
-        (void)hipGetLastError();
-      }
+    for (const GroupBuf& g : p->bufs) {
+      if (!p->eager_timed) break;
+      float a = 0.f, b = 0.f;
+      if (hipEventElapsedTime(&a, g.ev_ftc0, g.ev_ftc_t1) == hipSuccess) out->ftc_ms += a;
+      if (hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
+      (void)hipGetLastError();
     }
   }
   return FDAS_OK;
@@ -1383,6 +1443,7 @@ void fdas_plan_destroy(fdas_plan* p) {
   for (auto& b : p->bufs) free_group(b);
   for (char* r : p->regions) e->arena->free(r);
   for (char* m : p->inputs) e->arena->free(m);
+  if (p->small_all) e->arena->free(p->small_all);
   (void)hipHostFree(p->h_read);
   if (p->ev_start) (void)hipEventDestroy(p->ev_start);
   if (p->ev_end) (void)hipEventDestroy(p->ev_end);

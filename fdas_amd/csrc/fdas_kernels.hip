@@ -100,7 +100,7 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
         const uint32_t s = uint32_t(NT * a + tid);
         if (s >= overlap) {
           const uint32_t f = out0 + s - overlap;
-          if (f < n_bins) gst(out, f, v[a].x * v[a].x + v[a].y * v[a].y);
+          if (f < n_bins) gst_stream(out, f, v[a].x * v[a].x + v[a].y * v[a].y);
         }
       }
     }

@@ -46,6 +46,12 @@ __device__ __forceinline__ float2 gld2(const float2* p, size_t i) {
 __device__ __forceinline__ fdas_vf4 gld4(const float* p, size_t i) {
   return *(const FDAS_GLOBAL fdas_vf4*)(p + i);
 }
+// Streaming store: the data is read back only by a later kernel, so it is
+// written with the non-temporal hint and does not displace the cached
+// template spectra and twiddles.
+__device__ __forceinline__ void gst_stream(float* p, size_t i, float v) {
+  __builtin_nontemporal_store(v, (FDAS_GLOBAL float*)p + i);
+}
 __device__ __forceinline__ uint32_t gatomic_add(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_add((FDAS_GLOBAL uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
