@@ -43,6 +43,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000, calibrated
+# on a copy kernel) for the batched workload at N=2^21, T=85, tile 1024, 8 trials
+# per launch; profiles/batched_n2p21_t85_h8_tile1024_traffic.txt.
+MEASURED_TRAFFIC = {("ftconv_kernel<1024>", 8): 6845.5e6, ("hsum_kernel<8,true>", 8): 8939.1e6}
 FP32_PEAK_TFLOPS = 157.3
 N_METRIC = 1 << 21
 T_METRIC = 85
@@ -484,8 +488,11 @@ def main():
     kd = kernels[dominant]
     achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9
     ftc_name = [k for k in kernels if k.startswith("ftconv")][0]
+    traffic = MEASURED_TRAFFIC.get((dominant, trials_per_launch)) if batched is not None else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(traffic) if traffic else None,
+                "traffic_source": ("profiles/batched_n2p21_t85_h8_tile1024_traffic.txt (rocprofv3 --pmc, bytes per "
+                                   "launch)") if traffic else None,
                 "kernel": dominant, "kernel_ms": round(kd["ms"], 4), "algorithmic_bytes": int(kd["bytes"]),
                 "trials_per_launch": trials_per_launch,
                 "kernels": {k: {"ms": round(v["ms"], 4), "algorithmic_bytes": int(v["bytes"]),
