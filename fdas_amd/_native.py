@@ -41,6 +41,8 @@ class FdasCandidate(C.Structure):
 
 MAX_HARMONICS = 16
 REQ_DEVICE_SPECTRUM = 0x1
+FE_DEVICE_INPUT = 0x1
+FE_DEVICE_OUTPUT = 0x2
 PLAN_GRAPH = 0x1
 
 
@@ -159,6 +161,12 @@ SIGNATURES = {
     "fdas_status_string": (C.c_char_p, [_I]),
     "fdas_make_drift_bank": (_I, [_D, _u32, _u32, _F]),
     "fdas_reference_drifts": (_I, [_u32, C.c_double, C.c_double, C.c_double, C.c_double, _D]),
+    "fdas_read_tim": (_I, [C.c_char_p, _I, _F, C.c_uint64, C.POINTER(C.c_uint64), _D]),
+    "fdas_io_last_error": (C.c_char_p, []),
+    "fdas_frontend_create": (_I, [_u32, _I, C.POINTER(_P)]),
+    "fdas_frontend_destroy": (None, [_P]),
+    "fdas_frontend_run": (_I, [_P, _P, _u32, _P]),
+    "fdas_frontend_last_error": (C.c_char_p, [_P]),
     "fdas_engine_default_config": (None, [C.POINTER(EngineConfig)]),
     "fdas_engine_create": (_I, [C.POINTER(EngineConfig), LOG_FN, _P, C.POINTER(_P)]),
     "fdas_engine_destroy": (None, [_P]),

@@ -208,6 +208,33 @@ fdas_status fdas_reference_drifts(uint32_t n_templates, double max_accel, double
                                   double speed_of_light, double* out);
 
 /* ---------------------------------------------------------------------------
+ * Front end: time series -> spectrum (upstream of the reference's device path)
+ * ------------------------------------------------------------------------- */
+
+/* ft_conv.py:39-67 read_tim_file: a SIGPROC time series.  With read_header and a
+ * header ("HEADER_START" after a 4-byte length, within the first 512 bytes, up to
+ * "HEADER_END"), data_type must be 2 and nbits 32, and *t_samp receives tsamp
+ * (else 0).  float32 samples follow.  samples == NULL only counts them.  Host-only;
+ * the message of a failure is in fdas_io_last_error(). */
+fdas_status fdas_read_tim(const char* path, int read_header, float* samples, uint64_t capacity,
+                          uint64_t* n_samples, double* t_samp);
+const char* fdas_io_last_error(void);
+
+/* ft_conv.py:123-146 (the CXFT): X[f] = DFT(s)[f] / Ns for f < N from Ns = 2N
+ * real samples, FP32 on the device (the reference's scipy.fft.fft of float32
+ * samples is FP32 too).  N is a power of two, 2 .. 2^22.  No reference
+ * counterpart as a device call: the reference computes it offline in Python. */
+typedef struct fdas_frontend fdas_frontend;
+#define FDAS_FE_DEVICE_INPUT 0x1u  /* series is device memory */
+#define FDAS_FE_DEVICE_OUTPUT 0x2u /* spectrum is device memory */
+fdas_status fdas_frontend_create(uint32_t n_bins, int device, fdas_frontend** out);
+void fdas_frontend_destroy(fdas_frontend* fe);
+/* Blocking.  series: 2N floats; spectrum_c64: N complex64 (e.g. an input of
+ * fdas_set_input_device or an engine request with FDAS_REQ_DEVICE_SPECTRUM). */
+fdas_status fdas_frontend_run(fdas_frontend* fe, const float* series, uint32_t flags, void* spectrum_c64);
+const char* fdas_frontend_last_error(const fdas_frontend* fe);
+
+/* ---------------------------------------------------------------------------
  * Batched and served search (no counterpart in the reference, which runs one
  * trial per launch on one buffer set; these replace its A/B double buffering
  * for callers that search many dedispersion trials, FDAS.cpp:282-615).
