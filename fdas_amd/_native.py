@@ -153,6 +153,7 @@ SIGNATURES = {
     "fdas_print_stats": (None, [_P, _I, _I]),
     "fdas_print_events": (None, [_P, _I]),
     "fdas_get_stage_times": (_I, [_P, _I, _F]),
+    "fdas_quantile_thresholds": (_I, [_P, _u32, _I, _I, _F]),
     "fdas_get_size": (C.c_uint64, [_P, _I]),
     "fdas_device_count": (_I, []),
     "fdas_device_info": (_I, [_I, C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]),

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -918,6 +919,125 @@ extern "C" fdas_status fdas_retrieve_harmonic_plane(fdas_ctx* ctx, uint32_t k, f
   HIPCK(ctx, hipMemcpyAsync(plane + size_t(rb) * ctx->geo.n_bins, s.plane + size_t(rb) * ctx->geo.n_bins,
                             size_t(nr) * ctx->geo.n_bins * sizeof(float), hipMemcpyDeviceToHost, s.stream));
   HIPCK(ctx, hipStreamSynchronize(s.stream));
+  return FDAS_OK;
+}
+
+// hsum.py:125-131, the "mock-up" thresholds: thr_k = quantile(HP_k, 1 - n / M)
+// over the M = rows * N values of the part, linear interpolation between the
+// order statistics around q (M - 1), in FP64, rounded once to float32.  The two
+// order statistics are the (n+1)-th and n-th largest values.  Instead of
+// sorting M values: the (n+1)-th largest block maximum B_k bounds the (n+1)-th
+// largest value from below (n+1 blocks hold a value >= B_k), so one summing
+// pass with threshold just below B_k spills every value >= B_k, among them the
+// n+1 largest, and those few are sorted on the host.
+extern "C" fdas_status fdas_quantile_thresholds(fdas_ctx* ctx, uint32_t n_per_plane, fdas_part part, int set,
+                                                float* thresholds) {
+  CHECK_SET(ctx, set);
+  DevSet& s = ctx->set[set];
+  if (!thresholds) return fail(ctx, FDAS_EINVAL, "null thresholds pointer");
+  if (!valid_part(part)) return fail(ctx, FDAS_EINVAL, "invalid FOP part %d", int(part));
+  if (!s.fop_valid) return fail(ctx, FDAS_ESTATE, "no FOP in this buffer set (run the FT convolution first)");
+  uint32_t rb, nr;
+  part_rows(ctx, part, &rb, &nr);
+  const uint32_t N = ctx->geo.n_bins, H = ctx->cfg.n_harmonics;
+  const uint64_t M = uint64_t(nr) * N;
+  if (n_per_plane < 1 || n_per_plane >= M)
+    return fail(ctx, FDAS_EINVAL, "candidates per plane %u outside 1 .. %llu", n_per_plane,
+                (unsigned long long)(M - 1));
+  HIPCK(ctx, hipSetDevice(ctx->cfg.device));
+  fdas_status st = wait_staging(ctx, s.staged_hsum, s.hsum_pending);
+  if (st != FDAS_OK) return st;
+  HIPCK(ctx, hipStreamSynchronize(s.stream));
+  // 1. block maxima of every plane
+  const uint32_t nb = hmax_blocks(N, nr);
+  float* d_max = nullptr;
+  if ((st = dev_alloc(ctx, &d_max, size_t(H) * nb)) != FDAS_OK) return st;
+  std::vector<float> bmax(size_t(H) * nb);
+  hipError_t e = launch_hmax(s.fop, N, ctx->cfg.n_templates, rb, nr, H, d_max, s.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(bmax.data(), d_max, sizeof(float) * bmax.size(), hipMemcpyDeviceToHost,
+                                          s.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
+  (void)hipFree(d_max);
+  HIPCK(ctx, e);
+  // 2. per plane a threshold t with count(HP_k > t) >= n + 2 that fits the spill
+  //    lists: just below the (n+2)-th largest block maximum when there are that
+  //    many blocks, refined (or, without a bound, found) by bisection over the
+  //    bit patterns of non-negative floats, where each pass counts all planes.
+  const size_t want = size_t(std::min<uint64_t>(uint64_t(n_per_plane) + 2, M));  // a spare rank for rounding
+  auto value_of = [](int64_t b) { return b < 0 ? -INFINITY : __builtin_bit_cast(float, uint32_t(b)); };
+  std::vector<int64_t> lo_b(H, -1), hi_b(H, 0x7f800000);  // count(> lo) >= want, count(> hi) < want
+  for (uint32_t k = 0; k < H; ++k) {
+    std::vector<float> v(bmax.begin() + size_t(k) * nb, bmax.begin() + size_t(k + 1) * nb);
+    if (v.size() < want) continue;
+    std::nth_element(v.begin(), v.begin() + (want - 1), v.end(), std::greater<float>());
+    const float bound = std::nextafter(v[want - 1], -INFINITY);
+    lo_b[k] = bound < 0.f ? -1 : int64_t(__builtin_bit_cast(uint32_t, bound));
+  }
+  std::vector<float> thr(H);
+  std::vector<uint32_t> lo_count(H, 0);
+  bool probing = false;  // the last pass used a probe threshold for some plane
+  for (int pass = 0;; ++pass) {
+    // pass 0 measures the starting bounds; later passes bisect the planes that overflow
+    probing = false;
+    std::vector<int64_t> at(H);
+    for (uint32_t k = 0; k < H; ++k) {
+      at[k] = lo_b[k];
+      if (pass > 0 && lo_count[k] > ctx->spill) {
+        at[k] = lo_b[k] + (hi_b[k] - lo_b[k]) / 2;
+        probing = true;
+      }
+      thr[k] = value_of(at[k]);
+    }
+    if ((st = enqueue_hsum(ctx, s, part, thr.data())) != FDAS_OK) return st;
+    HIPCK(ctx, hipMemcpyAsync(s.h_counts, s.counts, sizeof(uint32_t) * H, hipMemcpyDeviceToHost, s.stream));
+    HIPCK(ctx, hipStreamSynchronize(s.stream));
+    bool all_fit = true;
+    for (uint32_t k = 0; k < H; ++k) {
+      const uint32_t c = s.h_counts[k];
+      if (pass == 0 || at[k] != lo_b[k]) {
+        if (c >= want) {
+          lo_b[k] = at[k];
+          lo_count[k] = c;
+        } else {
+          hi_b[k] = at[k];
+        }
+      }
+      if (lo_count[k] > ctx->spill) {
+        all_fit = false;
+        if (hi_b[k] - lo_b[k] <= 1)
+          return fail(ctx, FDAS_EOVERFLOW, "plane %u: %u values tie at the quantile, beyond the spill capacity %u",
+                      k + 1, lo_count[k], ctx->spill);
+      }
+    }
+    if (all_fit) break;
+  }
+  if (probing) {  // the spill lists hold a probe's entries: spill once more at the final thresholds
+    for (uint32_t k = 0; k < H; ++k) thr[k] = value_of(lo_b[k]);
+    if ((st = enqueue_hsum(ctx, s, part, thr.data())) != FDAS_OK) return st;
+    HIPCK(ctx, hipMemcpyAsync(s.h_counts, s.counts, sizeof(uint32_t) * H, hipMemcpyDeviceToHost, s.stream));
+    HIPCK(ctx, hipStreamSynchronize(s.stream));
+  }
+  s.cands_valid = false;  // the lists now belong to these thresholds, not to a search
+  const double q = 1.0 - double(n_per_plane) / double(M);
+  const double vi = q * double(M - 1);
+  const uint64_t below = uint64_t(std::floor(vi));  // ascending rank of the lower order statistic
+  const double g = vi - double(below);
+  for (uint32_t k = 0; k < H; ++k) {
+    const uint32_t c = s.h_counts[k];
+    if (c > ctx->spill)
+      return fail(ctx, FDAS_EOVERFLOW, "plane %u: %u values above the bound exceed the spill capacity %u", k + 1, c,
+                  ctx->spill);
+    std::vector<float> p(c);
+    if (c) HIPCK(ctx, hipMemcpy(p.data(), s.spill_pwr + size_t(k) * ctx->spill, sizeof(float) * c,
+                                hipMemcpyDeviceToHost));
+    // descending ranks of the two order statistics
+    const uint64_t r_lo = M - 1 - below, r_hi = below + 1 < M ? M - 2 - below : r_lo;
+    if (r_lo >= c) return fail(ctx, FDAS_EDEVICE, "plane %u: only %u values spilled, need %llu", k + 1, c,
+                               (unsigned long long)(r_lo + 1));
+    std::sort(p.begin(), p.end(), std::greater<float>());
+    const double lo = p[r_lo], hi = p[r_hi];
+    thresholds[k] = float(lo * (1.0 - g) + hi * g);
+  }
   return FDAS_OK;
 }
 

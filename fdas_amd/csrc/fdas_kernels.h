@@ -44,6 +44,12 @@ hipError_t launch_select(const SelectJob* d_jobs, uint32_t n_jobs, hipStream_t s
 hipError_t launch_hplane(const float* fop, uint32_t n_bins, uint32_t n_templates, uint32_t row_begin,
                          uint32_t n_rows, uint32_t k, float* out, hipStream_t stream);
 
+// Per-block maxima of HP_1 .. HP_H over a part (blocks of 256 bins x 8 rows),
+// out[k-1][block] for hmax_blocks(n_bins, n_rows) blocks per plane.
+uint32_t hmax_blocks(uint32_t n_bins, uint32_t n_rows);
+hipError_t launch_hmax(const float* fop, uint32_t n_bins, uint32_t n_templates, uint32_t row_begin, uint32_t n_rows,
+                       uint32_t n_harmonics, float* out, hipStream_t stream);
+
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 
 }  // namespace fdas

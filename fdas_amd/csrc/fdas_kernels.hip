@@ -410,6 +410,60 @@ hipError_t launch_hplane(const float* fop, uint32_t n_bins, uint32_t n_templates
   return hipGetLastError();
 }
 
+// Largest HP_k, k = 1..n_harmonics, of each block of 256 bins x kHmaxRows rows of
+// rows [row_begin, row_begin + n_rows): out[k-1][block], block = y * gridDim.x + x.
+// HP_k is summed in FP32 in k order, as in hsum_kernel, so the values are the
+// ones the summing kernel thresholds.
+constexpr uint32_t kHmaxRows = 8;
+
+__global__ __launch_bounds__(256) void hmax_kernel(const float* __restrict__ fop, uint32_t n_bins,
+                                                   uint32_t n_templates, uint32_t row_begin, uint32_t n_rows,
+                                                   uint32_t n_harmonics, float* __restrict__ out) {
+  __shared__ float red[kMaxHarmonics][256 / 64];
+  const uint32_t f = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t r0 = row_begin + blockIdx.y * kHmaxRows;
+  const uint32_t r1 = min(r0 + kHmaxRows, row_begin + n_rows);
+  const int c = int(n_templates / 2);
+  float mx[kMaxHarmonics];
+  for (uint32_t k = 0; k < kMaxHarmonics; ++k) mx[k] = -INFINITY;
+  if (f < n_bins) {
+    for (uint32_t r = r0; r < r1; ++r) {
+      const int t = int(r) - c;
+      float hp = gld(fop, size_t(r) * n_bins + f);
+      mx[0] = fmaxf(mx[0], hp);
+      for (uint32_t k = 2; k <= n_harmonics; ++k) {
+        hp += gld(fop, size_t(c + t / int(k)) * n_bins + f / k);
+        mx[k - 1] = fmaxf(mx[k - 1], hp);
+      }
+    }
+  }
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  for (uint32_t k = 0; k < n_harmonics; ++k) {
+    float v = mx[k];
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if (lane == 0) red[k][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < n_harmonics) {
+    const uint32_t k = threadIdx.x;
+    const float v = fmaxf(fmaxf(red[k][0], red[k][1]), fmaxf(red[k][2], red[k][3]));
+    gst(out, size_t(k) * (gridDim.x * gridDim.y) + blockIdx.y * gridDim.x + blockIdx.x, v);
+  }
+}
+
+uint32_t hmax_blocks(uint32_t n_bins, uint32_t n_rows) {
+  return (n_bins + 255) / 256 * ((n_rows + kHmaxRows - 1) / kHmaxRows);
+}
+
+hipError_t launch_hmax(const float* fop, uint32_t n_bins, uint32_t n_templates, uint32_t row_begin, uint32_t n_rows,
+                       uint32_t n_harmonics, float* out, hipStream_t stream) {
+  if (n_rows == 0 || n_bins == 0) return hipSuccess;
+  dim3 grid((n_bins + 255) / 256, (n_rows + kHmaxRows - 1) / kHmaxRows);
+  hipLaunchKernelGGL(hmax_kernel, grid, dim3(256), 0, stream, fop, n_bins, n_templates, row_begin, n_rows,
+                     n_harmonics, out);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Selection: keep the `cap` strongest detections of a plane (ties: smaller
 // (row, bin) key first), returned sorted by key.

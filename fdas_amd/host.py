@@ -257,6 +257,14 @@ class FDAS:
 
     # -- profiling (FDAS.h:74-76) -----------------------------------------
 
+    def quantile_thresholds(self, n_per_plane, which, ab):
+        """The reference's mock thresholds (hsum.py:125-131) from the FOP in set ab: about
+        n_per_plane candidates per plane.  Returns float32[H], or None on failure."""
+        thr = np.zeros(self.cfg.n_harmonics, np.float32)
+        if not self._ok(self._lib.fdas_quantile_thresholds(self._ctx, int(n_per_plane), which, ab, N.fptr(thr))):
+            return None
+        return thr
+
     def print_stats(self, ab, reset=False):
         self._lib.fdas_print_stats(self._ctx, ab, 1 if reset else 0)
 

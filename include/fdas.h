@@ -178,6 +178,13 @@ fdas_status fdas_retrieve_harmonic_plane(fdas_ctx* ctx, uint32_t k, float* plane
 void fdas_print_stats(fdas_ctx* ctx, int set, int reset);
 void fdas_print_events(fdas_ctx* ctx, int set);
 
+/* hsum.py:125-131: the reference's "mock-up" thresholds, thr_k = quantile(HP_k,
+ * 1 - n / M) over the M values of plane k in the part (linear interpolation,
+ * FP64, rounded to float32), giving about n candidates per plane, for k = 1..H,
+ * from the FOP in `set` (after fdas_perform_ft_convolution or fdas_inject_fop).
+ * Overwrites the set's candidate lists. */
+fdas_status fdas_quantile_thresholds(fdas_ctx* ctx, uint32_t n_per_plane, fdas_part part, int set, float* thresholds);
+
 /* Device durations in ms of the last run on `set` (HIP events on the set's
  * stream): [0] input, [1] FT-convolution kernel, [2] harmonic-summing kernel,
  * [3] candidate selection, [4] candidate read-back, [5] total. */

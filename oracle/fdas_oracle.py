@@ -350,12 +350,20 @@ def noise_thresholds(x: np.ndarray, n_harmonics: int, sigma: float = 6.0) -> np.
     return ((k + sigma * np.sqrt(k)) * p).astype(np.float32)
 
 
-def quantile_thresholds(hps: np.ndarray, n_cands_per_plane: int) -> np.ndarray:
-    """Mock thresholds thr_k = quantile(HP_k, 1 - n/FOP.size) (hsum.py:125-131)."""
+def quantile_thresholds(hps: np.ndarray, n_cands_per_plane: int, rows=None) -> np.ndarray:
+    """Mock thresholds thr_k = quantile(HP_k, 1 - n/M) (hsum.py:125-131), M values per plane.
+
+    Linear interpolation at q (M - 1) between order statistics, evaluated in FP64
+    and rounded once to float32 (what the reference's NumPy 1.19 computes for a
+    float32 plane; NumPy 2 would round the index to float32 first).  ``rows``
+    restricts the planes to a FOP part.
+    """
     hps = np.asarray(hps)
+    if rows is not None:
+        hps = hps[:, list(rows)]
     size = hps[0].size
-    return np.array([np.quantile(hps[h], 1.0 - n_cands_per_plane / size) for h in range(hps.shape[0])],
-                    dtype=np.float32)
+    return np.array([np.quantile(hps[h].astype(np.float64), 1.0 - n_cands_per_plane / size)
+                     for h in range(hps.shape[0])], dtype=np.float32)
 
 
 # ---------------------------------------------------------------------------

@@ -383,3 +383,26 @@ def test_selection_with_massive_ties(cap):
     planes, det = wide_by_plane(f, FDAS.A, 8)
     assert_lists_match(planes, det, sel, thr, cap, 42, exact=True)
     f.close()
+
+
+# -- the reference's mock-up thresholds (hsum.py:125-131) ------------------------
+
+@pytest.mark.parametrize("part", list(PARTS))
+@pytest.mark.parametrize("n_templates,n_harmonics,n_cands", [(85, 8, 64), (9, 4, 10), (169, 16, 500)])
+def test_quantile_thresholds_on_device(part, n_templates, n_harmonics, n_cands):
+    n = 6007
+    rng = np.random.default_rng(7 + part + n_templates)
+    fop = rng.exponential(1.0, size=(n_templates, n)).astype(np.float32)
+    fop[n_templates // 2, 3000] = 200.0  # a line that dominates a few order statistics
+    f = make(n, n_templates=n_templates, n_harmonics=n_harmonics, cands_per_harmonic=256)
+    assert f.inject_FOP(fop, FDAS.A), f.last_error
+    thr = f.quantile_thresholds(n_cands, part, FDAS.A)
+    assert thr is not None, f.last_error
+    rows = O.part_rows(n_templates, PARTS[part])
+    hps = O.harmonic_planes(fop, n_harmonics, rows, dtype=np.float32)
+    ref = O.quantile_thresholds(hps, n_cands, rows)
+    assert np.array_equal(thr, ref)
+    # about n candidates per plane: the threshold lies between the n-th and (n+1)-th largest value
+    for k in range(n_harmonics):
+        assert n_cands - 1 <= int(np.sum(hps[k][list(rows)] > thr[k])) <= n_cands
+    f.close()

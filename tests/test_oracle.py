@@ -218,3 +218,20 @@ def test_golden_small_is_consistent(golden_small):
     assert np.allclose(g["fop"][42], np.abs(g["spectrum"].astype(np.complex128)) ** 2, rtol=1e-6)
     for h in range(8):
         assert np.all(g[f"powers_{h + 1}"] > g["thresholds"][h])
+
+
+def test_quantile_thresholds_definition():
+    """Linear interpolation between order statistics at q (M - 1), FP64, one rounding to float32."""
+    rng = np.random.default_rng(11)
+    hps = rng.exponential(size=(3, 7, 301)).astype(np.float32)
+    for n in (1, 10, 64, 2106):
+        thr = O.quantile_thresholds(hps, n)
+        for h in range(3):
+            v = np.sort(hps[h].ravel().astype(np.float64))
+            m = v.size
+            vi = (1.0 - n / m) * (m - 1)
+            i = int(np.floor(vi))
+            g = vi - i
+            assert thr[h] == np.float32(v[i] * (1 - g) + v[min(i + 1, m - 1)] * g)
+    sub = O.quantile_thresholds(hps, 10, rows=range(2, 5))
+    assert np.array_equal(sub, O.quantile_thresholds(hps[:, 2:5], 10))
