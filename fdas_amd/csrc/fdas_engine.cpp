@@ -691,7 +691,7 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
       if (tile_usable(s, bank.max_extent)) tiles.push_back(s);
   }
   std::vector<uint32_t> rpis = e->cfg.rows_per_item ? std::vector<uint32_t>{e->cfg.rows_per_item}
-                                                    : std::vector<uint32_t>{8, 16, 32};
+                                                    : std::vector<uint32_t>{8, 16, 32, 64};
   std::vector<uint32_t> hrows = e->cfg.hsum_rows ? std::vector<uint32_t>{e->cfg.hsum_rows}
                                                  : std::vector<uint32_t>{6, kHsumRowsPerBlock, 22};
   // Defaults used when nothing needs timing.
