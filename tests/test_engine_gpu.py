@@ -244,3 +244,24 @@ def test_request_validation(banks):
         # 1500-tap templates need a 4096-point tile; a 2048 engine refuses them
         with pytest.raises(EngineError):
             eng.add_bank(fdas_amd.make_drift_bank(synth.drift_grid(5, 300.0), 1500))
+
+
+@pytest.mark.parametrize("n,T,H,part", [(4099, 9, 1, 2), (1001, 85, 3, 0), (20011, 169, 16, 1), (64, 85, 8, 2)])
+def test_edge_shapes_against_oracle(banks, n, T, H, part):
+    """N not a multiple of 4 (scalar loads), one harmonic, tiny N, wide banks, each part."""
+    bank = banks.get(T)
+    if bank is None:
+        bank = bank_for(T)
+    cap = 32
+    x = spectrum(n, 77 + n, f0_frac=0.37)
+    thr = synth.noise_thresholds(x, H)
+    with Engine(cands_per_harmonic=cap, log=quiet) as eng:
+        b = eng.add_bank(bank)
+        (r,) = eng.run([Trial(x, b, thr, part)])
+        assert r.status == 0
+        _, _, sel = O.run(x, bank, thr, H, cap, PART_NAMES[part])
+        assert_lists_match(planes_of(r, T), r.n_detected, sel, thr, cap, T // 2)
+        # served: the same result
+        eng.submit(Trial(x, b, thr, part, tag=5))
+        (s,) = eng.drain()
+        assert_same(s, r)
