@@ -279,7 +279,7 @@ fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* 
   HIPCK(ctx, launch_zero_u32(s.counts, H * (2 + kSelectBins), s.stream));
   const uint32_t blocks = (ctx->geo.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
   if ((st = record(ctx, s, EV_HSUM_K0)) != FDAS_OK) return st;
-  HIPCK(ctx, launch_hsum(s.d_hsum, 1, blocks, H, (ctx->geo.n_bins & 3u) == 0, s.stream));
+  HIPCK(ctx, launch_hsum(s.d_hsum, 1, blocks, H, hsum_vec_ok(ctx->geo.n_bins), s.stream));
   if ((st = record(ctx, s, EV_HSUM_K1)) != FDAS_OK) return st;
   HIPCK(ctx, launch_select(s.d_sel, H, s.stream));
   return FDAS_OK;

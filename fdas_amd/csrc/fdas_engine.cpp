@@ -497,7 +497,7 @@ fdas_status build_group(fdas_engine* e, GroupBuf& g, const std::vector<Trial*>& 
       j.counts = reinterpret_cast<uint32_t*>(sm + l.counts);
       for (uint32_t k = 0; k < H; ++k) j.thr[k] = t->req.thresholds[k];
       L.n_blocks += (j.n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
-      L.vec = L.vec && (j.n_bins & 3u) == 0;
+      L.vec = L.vec && hsum_vec_ok(j.n_bins);
       for (uint32_t p = 0; p < H; ++p) {
         SelectJob& q = g.h_sel[js++];
         q.spill_key = j.spill_key + size_t(p) * spill;
@@ -780,7 +780,7 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
               e,
               [&]() -> fdas_status {
                 EHIP(e, launch_zero_u32(j.counts, 1, e->s_ftc));
-                EHIP(e, launch_hsum(g.d_hsum, 1, blocks, 8, (n_bins & 3u) == 0, e->s_ftc));
+                EHIP(e, launch_hsum(g.d_hsum, 1, blocks, 8, hsum_vec_ok(n_bins), e->s_ftc));
                 return FDAS_OK;
               },
               2, &ms);

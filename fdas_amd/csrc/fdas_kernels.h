@@ -10,8 +10,10 @@
 namespace fdas {
 
 constexpr int kHsumThreads = 256;
-constexpr int kHsumBinsPerThread = 4;
+constexpr int kHsumBinsPerThread = 8;
 constexpr int kHsumBinsPerBlock = kHsumThreads * kHsumBinsPerThread;  // bins one HSUM workgroup covers
+// The 16-byte-load form of the summing kernel needs whole 8-bin groups.
+inline bool hsum_vec_ok(uint32_t n_bins) { return n_bins % kHsumBinsPerThread == 0; }
 constexpr uint32_t kHsumRowsPerBlock = 11;  // target rows per HSUM workgroup (bounds the dense-hit tail)
 constexpr int kHsumWaveStage = 64;        // LDS staging entries per plane per wave before a global flush
 constexpr int kSelectThreads = 1024;

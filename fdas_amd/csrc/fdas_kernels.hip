@@ -204,79 +204,118 @@ __device__ __forceinline__ uint32_t find_job_hsum(const HsumJob* __restrict__ jo
   return lo;
 }
 
-// Every FOP value one thread needs for one row: its 4 own bins and, for each
-// harmonic k >= 2, the (at most two) sub-harmonic bins (f0 .. f0+3) / k of
-// row trunc(t / k).  f0 is a multiple of 4, so (f0 .. f0+3) / 2 spans two bins.
+// A thread sums 8 consecutive bins f0 .. f0+7 of a row, f0 a multiple of 8.
+// For harmonic k their sub-harmonic bins (f0 + b) / k are the hsum_span(k)
+// consecutive bins from f0 / k: f0 mod k is a multiple of g = gcd(8, k), at
+// most k - g, so the last one is at most (k - g + 7) / k bins further on.
+__host__ __device__ constexpr int hsum_gcd8(int k) { return (k % 8 == 0) ? 8 : (k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : 1; }
+__host__ __device__ constexpr int hsum_span(int k) { return 1 + (k - hsum_gcd8(k) + 7) / k; }
+constexpr int kHsumMaxSpan = 4;  // hsum_span(2) = hsum_span(3) = 4 is the largest
+static_assert(kHsumBinsPerThread == 8, "the sub-harmonic spans assume 8 bins per thread");
+
+// Every FOP value one thread needs for one row: its 8 own bins and, for each
+// harmonic k >= 2, the hsum_span(k) sub-harmonic bins of row trunc(t / k).
 template <int H>
 struct RowLoads {
   float m[kHsumBinsPerThread];
-  float lo[H];
-  float hi[H];
+  float s[H][kHsumMaxSpan];
 };
 
-// Branch-free: H is a compile-time constant and out-of-range bins are clamped,
-// so the compiler can count the loads in flight exactly (a load issued under a
+// Branch-free: H is a compile-time constant and the indexes are clamped, so
+// the compiler can count the loads in flight exactly (a load issued under a
 // condition makes it drain every outstanding load at the next use).
+//  VEC (n_bins % 8 == 0): f0 is clamped to the last 8 bins, so every index is
+//    in range; the own bins are two 16-byte loads, and so are the k = 2 bins
+//    (f0 / 2 is a multiple of 4); the k = 4 pair is one 8-byte load.
+//  otherwise: one load per value, each index clamped to the last bin, and the
+//    own bins past the end read as -inf, so no sum of theirs passes a threshold.
 template <int H, bool VEC>
 __device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __restrict__ fop, uint32_t n_bins, int c,
                                               uint32_t r, uint32_t f0) {
   const float* __restrict__ row = fop + size_t(r) * n_bins;
   const uint32_t last = n_bins - 1;
   if constexpr (VEC) {
-    // n_bins % 4 == 0: the whole quad is either in range or entirely past the end
-    const fdas_vf4 q = gld4(row, min(f0, n_bins - 4));
-    L.m[0] = q.x;
-    L.m[1] = q.y;
-    L.m[2] = q.z;
-    L.m[3] = q.w;
+    const fdas_vf4 q0 = gld4(row, f0);
+    const fdas_vf4 q1 = gld4(row, f0 + 4);
+    L.m[0] = q0.x; L.m[1] = q0.y; L.m[2] = q0.z; L.m[3] = q0.w;
+    L.m[4] = q1.x; L.m[5] = q1.y; L.m[6] = q1.z; L.m[7] = q1.w;
   } else {
 #pragma unroll
-    for (int b = 0; b < kHsumBinsPerThread; ++b) L.m[b] = gld(row, min(f0 + uint32_t(b), last));
+    for (int b = 0; b < kHsumBinsPerThread; ++b) {
+      const float v = gld(row, min(f0 + uint32_t(b), last));
+      L.m[b] = (f0 + uint32_t(b) < n_bins) ? v : -INFINITY;
+    }
   }
   const int t = int(r) - c;
-  const uint32_t fa = min(f0, last), fb = min(f0 + 3u, last);
 #pragma unroll
   for (int k = 2; k <= H; ++k) {
     const float* __restrict__ srow = fop + size_t(c + t / k) * n_bins;
-    L.lo[k - 1] = gld(srow, fa / uint32_t(k));
-    L.hi[k - 1] = gld(srow, fb / uint32_t(k));
+    const uint32_t s0 = f0 / uint32_t(k);
+    if (VEC && k == 2) {
+      const fdas_vf4 q = gld4(srow, s0);
+      L.s[k - 1][0] = q.x; L.s[k - 1][1] = q.y; L.s[k - 1][2] = q.z; L.s[k - 1][3] = q.w;
+    } else if (VEC && k == 4) {
+      const float2 q = gld2(reinterpret_cast<const float2*>(srow), s0 / 2);
+      L.s[k - 1][0] = q.x; L.s[k - 1][1] = q.y;
+    } else {
+#pragma unroll
+      for (int j = 0; j < hsum_span(k); ++j) L.s[k - 1][j] = gld(srow, VEC ? s0 + uint32_t(j) : min(s0 + uint32_t(j), last));
+    }
   }
 }
 
-// split[k-1] = first of the thread's 4 bins whose sub-harmonic bin (f0+b)/k is
-// the upper one: k - (f0 mod k) (f0 is a multiple of 4, so the 4 bins span at
-// most two sub-harmonic bins for every k >= 2).  Per thread, fixed for all rows.
+// Bin b of the thread takes sub-harmonic value (f0 mod k + b) / k of its span:
+// b / k when f0 mod k < k (b / k + 1) - b, else the next one.  rk[k-1] = f0 mod k.
+template <int K>
+__device__ __forceinline__ float hsum_sub(const float (&s)[kHsumMaxSpan], uint32_t rk, int b) {
+  const int lo = b / K;
+  const int edge = K * (lo + 1) - b;  // f0 mod k at which bin b moves to the next value
+  if (edge > K - hsum_gcd8(K)) return s[lo];  // no f0 mod k reaches it
+  return rk >= uint32_t(edge) ? s[lo + 1] : s[lo];
+}
+
+template <int H, int K>
+__device__ __forceinline__ void hsum_add(float (&hp)[kHsumBinsPerThread], const RowLoads<H>& L, uint32_t rk) {
+#pragma unroll
+  for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] += hsum_sub<K>(L.s[K - 1], rk, b);
+}
+
+template <int H, int K = 2>
+__device__ __forceinline__ void hsum_add_k(int k, float (&hp)[kHsumBinsPerThread], const RowLoads<H>& L,
+                                           const uint32_t (&rk)[H]) {
+  if constexpr (K <= H) {
+    if (k == K) hsum_add<H, K>(hp, L, rk[K - 1]);
+    else hsum_add_k<H, K + 1>(k, hp, L, rk);
+  }
+}
+
+// One test per plane for the common case: the largest of the 8 sums against
+// the threshold; only a wave with a hit looks at the bins one by one.
+// thr[] is +inf for a thread whose bins all lie past the end.
 template <int H>
 __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
-                                         uint32_t f0, uint32_t n_valid, const uint32_t (&split)[H],
-                                         const float (&thr)[H], uint32_t (&wn)[H], uint32_t wave) {
+                                         uint32_t f0, const uint32_t (&rk)[H], const float (&thr)[H],
+                                         uint32_t (&wn)[H], uint32_t wave) {
   float hp[kHsumBinsPerThread];
 #pragma unroll
   for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] = L.m[b];
 #pragma unroll
   for (int k = 1; k <= H; ++k) {
-    if (k > 1) {
-#pragma unroll
-      for (int b = 0; b < kHsumBinsPerThread; ++b) hp[b] += (uint32_t(b) >= split[k - 1]) ? L.hi[k - 1] : L.lo[k - 1];
-    }
-    bool hit[kHsumBinsPerThread];
-    bool any = false;
+    if (k > 1) hsum_add_k<H>(k, hp, L, rk);
+    const float mx = fmaxf(fmaxf(fmaxf(hp[0], hp[1]), fmaxf(hp[2], hp[3])),
+                           fmaxf(fmaxf(hp[4], hp[5]), fmaxf(hp[6], hp[7])));
+    if (__ballot(mx > thr[k - 1]) == 0) continue;
 #pragma unroll
     for (int b = 0; b < kHsumBinsPerThread; ++b) {
-      hit[b] = uint32_t(b) < n_valid && hp[b] > thr[k - 1];
-      any |= hit[b];
-    }
-    if (__ballot(any) == 0) continue;
-#pragma unroll
-    for (int b = 0; b < kHsumBinsPerThread; ++b) {
-      const uint64_t mask = __ballot(hit[b]);
+      const bool hit = hp[b] > thr[k - 1];
+      const uint64_t mask = __ballot(hit);
       if (mask == 0) continue;
       const uint32_t cnt = uint32_t(__builtin_popcountll(mask));
       if (wn[k - 1] + cnt > uint32_t(kHsumWaveStage)) {
         wave_flush(st, job, wave, uint32_t(k - 1), wn[k - 1]);
         wn[k - 1] = 0;
       }
-      if (hit[b]) {
+      if (hit) {
         const uint32_t pos = wn[k - 1] + lanes_below(mask);
         st.key[wave][k - 1][pos] = (r << kKeyBinBits) | (f0 + uint32_t(b));
         st.pwr[wave][k - 1][pos] = hp[b];
@@ -287,10 +326,10 @@ __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, c
 }
 
 // HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] in FP32, in k order (hsum.py:331);
-// strict HP_k > thr_k (detect.cl.mako:81-83).  One workgroup covers 1024 bins
-// of every row of its part, each thread 4 consecutive bins.  Rows are processed
-// in pairs with two register buffers: the loads of row r+1 are in flight while
-// row r is summed and thresholded.
+// strict HP_k > thr_k (detect.cl.mako:81-83).  One workgroup covers 2048 bins
+// of a chunk of rows of its part, each thread 8 consecutive bins.  Rows are
+// processed in pairs with two register buffers: the loads of row r+1 are in
+// flight while row r is summed and thresholded.
 template <int H, bool VEC>
 __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __restrict__ jobs, uint32_t n_jobs) {
   __shared__ HsumStage<H> st;
@@ -304,26 +343,27 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   const uint32_t local = blockIdx.x - job.block_begin;
   const uint32_t chunk = local % job.n_row_chunks;  // row chunks of one bin block run side by side
   const uint32_t f0 = (local / job.n_row_chunks) * uint32_t(kHsumBinsPerBlock) + uint32_t(tid * kHsumBinsPerThread);
+  // VEC: threads past the end read (and discard) the last 8 bins; both values are multiples of 8
+  const uint32_t fl = VEC ? min(f0, (n_bins - 8u) & ~7u) : f0;
   const uint32_t r_begin = job.row_begin + chunk * job.rows_per_block;
   const uint32_t r_last = min(r_begin + job.rows_per_block, job.row_begin + job.n_rows) - 1;
   float thr[H];
-  uint32_t wn[H], split[H];
+  uint32_t wn[H], rk[H];
 #pragma unroll
   for (int k = 0; k < H; ++k) {
-    thr[k] = job.thr[k];
+    thr[k] = f0 < n_bins ? job.thr[k] : INFINITY;
     wn[k] = 0;
-    split[k] = uint32_t(k + 1) - f0 % uint32_t(k + 1);
+    rk[k] = fl % uint32_t(k + 1);
   }
-  const uint32_t n_valid = f0 < n_bins ? min(n_bins - f0, uint32_t(kHsumBinsPerThread)) : 0u;
 
   RowLoads<H> a, b;
-  hsum_load_row<H, VEC>(a, fop, n_bins, c, r_begin, f0);
+  hsum_load_row<H, VEC>(a, fop, n_bins, c, r_begin, fl);
   for (uint32_t r = r_begin;; r += 2) {
-    hsum_load_row<H, VEC>(b, fop, n_bins, c, min(r + 1, r_last), f0);
-    hsum_row<H>(st, job, a, r, f0, n_valid, split, thr, wn, wave);
+    hsum_load_row<H, VEC>(b, fop, n_bins, c, min(r + 1, r_last), fl);
+    hsum_row<H>(st, job, a, r, fl, rk, thr, wn, wave);
     if (r + 1 > r_last) break;
-    hsum_load_row<H, VEC>(a, fop, n_bins, c, min(r + 2, r_last), f0);
-    hsum_row<H>(st, job, b, r + 1, f0, n_valid, split, thr, wn, wave);
+    hsum_load_row<H, VEC>(a, fop, n_bins, c, min(r + 2, r_last), fl);
+    hsum_row<H>(st, job, b, r + 1, fl, rk, thr, wn, wave);
     if (r + 2 > r_last) break;
   }
   // Flush what is left: one global atomic per plane for the whole workgroup.
