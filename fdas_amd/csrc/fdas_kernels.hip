@@ -290,11 +290,12 @@ __device__ __forceinline__ void hsum_add_k(int k, float (&hp)[kHsumBinsPerThread
 }
 
 // One test per plane for the common case: the largest of the 8 sums against
-// the threshold; only a wave with a hit looks at the bins one by one.
-// thr[] is +inf for a thread whose bins all lie past the end.
+// the threshold; only a wave with a hit looks at the bins one by one.  The
+// thresholds are wave-uniform; `live` is false for a thread whose bins all lie
+// past the end.
 template <int H>
 __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
-                                         uint32_t f0, const uint32_t (&rk)[H], const float (&thr)[H],
+                                         uint32_t f0, const uint32_t (&rk)[H], const float (&thr)[H], bool live,
                                          uint32_t (&wn)[H], uint32_t wave) {
   float hp[kHsumBinsPerThread];
 #pragma unroll
@@ -304,10 +305,10 @@ __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, c
     if (k > 1) hsum_add_k<H>(k, hp, L, rk);
     const float mx = fmaxf(fmaxf(fmaxf(hp[0], hp[1]), fmaxf(hp[2], hp[3])),
                            fmaxf(fmaxf(hp[4], hp[5]), fmaxf(hp[6], hp[7])));
-    if (__ballot(mx > thr[k - 1]) == 0) continue;
+    if (__ballot(live && mx > thr[k - 1]) == 0) continue;
 #pragma unroll
     for (int b = 0; b < kHsumBinsPerThread; ++b) {
-      const bool hit = hp[b] > thr[k - 1];
+      const bool hit = live && hp[b] > thr[k - 1];
       const uint64_t mask = __ballot(hit);
       if (mask == 0) continue;
       const uint32_t cnt = uint32_t(__builtin_popcountll(mask));
@@ -351,19 +352,20 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   uint32_t wn[H], rk[H];
 #pragma unroll
   for (int k = 0; k < H; ++k) {
-    thr[k] = f0 < n_bins ? job.thr[k] : INFINITY;
+    thr[k] = job.thr[k];
     wn[k] = 0;
     rk[k] = fl % uint32_t(k + 1);
   }
 
+  const bool live = f0 < n_bins;
   RowLoads<H> a, b;
   hsum_load_row<H, VEC>(a, fop, n_bins, c, r_begin, fl);
   for (uint32_t r = r_begin;; r += 2) {
     hsum_load_row<H, VEC>(b, fop, n_bins, c, min(r + 1, r_last), fl);
-    hsum_row<H>(st, job, a, r, fl, rk, thr, wn, wave);
+    hsum_row<H>(st, job, a, r, fl, rk, thr, live, wn, wave);
     if (r + 1 > r_last) break;
     hsum_load_row<H, VEC>(a, fop, n_bins, c, min(r + 2, r_last), fl);
-    hsum_row<H>(st, job, b, r + 1, fl, rk, thr, wn, wave);
+    hsum_row<H>(st, job, b, r + 1, fl, rk, thr, live, wn, wave);
     if (r + 2 > r_last) break;
   }
   // Flush what is left: one global atomic per plane for the whole workgroup.
