@@ -169,6 +169,20 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
+// Exclusive prefix sum over the wave of a per-lane count v < 16, bit-sliced:
+// one ballot per bit.  `total` receives the wave's sum (wave-uniform).
+__device__ __forceinline__ uint32_t wave_prefix_small(uint32_t v, uint32_t& total) {
+  uint32_t excl = 0;
+  total = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint64_t m = __ballot((v >> j) & 1u);
+    excl += lanes_below(m) << j;
+    total += uint32_t(__builtin_popcountll(m)) << j;
+  }
+  return excl;
+}
+
 // Per-wave detection staging: each wave owns kHsumWaveStage LDS slots per
 // plane and a wave-uniform fill count.  A full stage is flushed by the wave
 // alone (one global atomic for up to 64 entries); what is left at the end is
@@ -306,22 +320,43 @@ __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, c
     const float mx = fmaxf(fmaxf(fmaxf(hp[0], hp[1]), fmaxf(hp[2], hp[3])),
                            fmaxf(fmaxf(hp[4], hp[5]), fmaxf(hp[6], hp[7])));
     if (__ballot(live && mx > thr[k - 1]) == 0) continue;
+    // The wave's hits in lane order, each lane's in bin order: one prefix sum
+    // over the per-lane counts places them all.
+    uint32_t bits = 0;
 #pragma unroll
-    for (int b = 0; b < kHsumBinsPerThread; ++b) {
-      const bool hit = live && hp[b] > thr[k - 1];
-      const uint64_t mask = __ballot(hit);
-      if (mask == 0) continue;
-      const uint32_t cnt = uint32_t(__builtin_popcountll(mask));
-      if (wn[k - 1] + cnt > uint32_t(kHsumWaveStage)) {
-        wave_flush(st, job, wave, uint32_t(k - 1), wn[k - 1]);
-        wn[k - 1] = 0;
+    for (int b = 0; b < kHsumBinsPerThread; ++b) bits |= (live && hp[b] > thr[k - 1]) ? (1u << b) : 0u;
+    uint32_t total;
+    uint32_t pos = wave_prefix_small(uint32_t(__builtin_popcount(bits)), total);
+    if (wn[k - 1] + total > uint32_t(kHsumWaveStage)) {
+      wave_flush(st, job, wave, uint32_t(k - 1), wn[k - 1]);
+      wn[k - 1] = 0;
+    }
+    if (total > uint32_t(kHsumWaveStage)) {
+      // more than a stage holds: straight to the spill list (entries past its end are counted, not kept)
+      uint32_t g = 0;
+      if (lane_id() == 0) g = gatomic_add(job.counts + (k - 1), total);
+      g = __shfl(g, 0) + pos;
+#pragma unroll
+      for (int b = 0; b < kHsumBinsPerThread; ++b) {
+        if ((bits >> b) & 1u) {
+          if (g < job.spill_cap) {
+            gst(job.spill_key, size_t(k - 1) * job.spill_cap + g, (r << kKeyBinBits) | (f0 + uint32_t(b)));
+            gst(job.spill_pwr, size_t(k - 1) * job.spill_cap + g, hp[b]);
+          }
+          ++g;
+        }
       }
-      if (hit) {
-        const uint32_t pos = wn[k - 1] + lanes_below(mask);
-        st.key[wave][k - 1][pos] = (r << kKeyBinBits) | (f0 + uint32_t(b));
-        st.pwr[wave][k - 1][pos] = hp[b];
+    } else {
+      pos += wn[k - 1];
+#pragma unroll
+      for (int b = 0; b < kHsumBinsPerThread; ++b) {
+        if ((bits >> b) & 1u) {
+          st.key[wave][k - 1][pos] = (r << kKeyBinBits) | (f0 + uint32_t(b));
+          st.pwr[wave][k - 1][pos] = hp[b];
+          ++pos;
+        }
       }
-      wn[k - 1] += cnt;
+      wn[k - 1] += total;
     }
   }
 }

@@ -249,6 +249,27 @@ def test_wide_only_configuration_rejects_reference_layout():
     f.close()
 
 
+@pytest.mark.parametrize("n", [4096, 4001])
+def test_hsum_dense_detections_bit_exact(n):
+    """Thresholds at half the plane mean: most bins fire, so a wave row holds more
+    hits than its LDS stage and they go straight to the spill list (16-byte-load
+    form at n = 4096, per-bin loads at n = 4001)."""
+    rng = np.random.default_rng(n)
+    fop = rng.exponential(1.0, size=(85, n)).astype(np.float32)
+    f = make(n, cands_per_harmonic=256, spill_per_harmonic=85 * n)  # room for every bin
+    assert f.inject_FOP(fop, FDAS.A), f.last_error
+    thr = np.array([0.5 * k for k in range(1, 9)], dtype=np.float32)
+    assert f.perform_harmonic_summing(thr, FDAS.AllAccelerations, FDAS.A), f.last_error
+    rows = O.part_rows(85, "all")
+    hps = O.harmonic_planes(fop, 8, rows, dtype=np.float32)
+    det = O.detect(hps, thr, rows)
+    assert min(d.powers.size for d in det) > 85 * n // 4  # dense in every plane
+    sel = O.select(det, 256)
+    planes, gdet = wide_by_plane(f, FDAS.A, 8)
+    assert_lists_match(planes, gdet, sel, thr, 256, 42, exact=True)
+    f.close()
+
+
 # -- drivers of the reference test (main_gtest.cpp:315-437) ---------------------
 
 def _blocking_lists(x, bank, thr, part):
