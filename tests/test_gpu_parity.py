@@ -175,8 +175,8 @@ def test_golden_small(golden_small):
 
 @pytest.mark.parametrize("part", list(PARTS))
 @pytest.mark.parametrize("n_templates,n_harmonics", [(85, 8), (169, 16), (9, 8)])
-def test_hsum_bit_exact_on_injected_fop(part, n_templates, n_harmonics):
-    n = 6007
+@pytest.mark.parametrize("n", [6007, 6008])  # per-bin loads / 16-byte loads (n % 8 == 0)
+def test_hsum_bit_exact_on_injected_fop(part, n_templates, n_harmonics, n):
     rng = np.random.default_rng(n_templates + part)
     fop = rng.exponential(1.0, size=(n_templates, n)).astype(np.float32)
     fop[n_templates // 2 + 3, 1234] = 60.0  # a strong line
