@@ -94,13 +94,15 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
 #pragma unroll
         for (int s = 0; s < 16; ++s) hn[s] = gld2(h, size_t(s * NT));
       }
-      float* __restrict__ out = job->fop + size_t(r) * n_bins;
+      // Slot a of this thread is FOP bin out0 + NT*a + tid - overlap: one base
+      // address per row and a constant offset per slot (the store's immediate).
+      float* __restrict__ out = job->fop + size_t(r) * n_bins + (ptrdiff_t(out0) + tid - ptrdiff_t(overlap));
 #pragma unroll
       for (int a = 0; a < 16; ++a) {
         const uint32_t s = uint32_t(NT * a + tid);
         if (s >= overlap) {
           const uint32_t f = out0 + s - overlap;
-          if (f < n_bins) gst_stream(out, f, v[a].x * v[a].x + v[a].y * v[a].y);
+          if (f < n_bins) gst_stream(out, size_t(NT * a), v[a].x * v[a].x + v[a].y * v[a].y);
         }
       }
     }
