@@ -75,9 +75,10 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
     const float2* __restrict__ hbase = static_cast<const float2*>(job->hperm);
     const uint32_t out0 = tile * payload;  // FOP bin of tile sample s = overlap
 
-    // The template spectrum of the next row is loaded before this row's FOP
-    // stores: loads and stores share one in-order completion counter, so a
-    // load issued after the stores would wait for them too.
+    // The template spectrum of the next row is loaded as soon as this row's has
+    // been used, so the loads have a whole inverse transform to complete (and
+    // they precede this row's FOP stores: loads and stores share one in-order
+    // completion counter, so a load issued after the stores would wait for them).
     float2 hn[16];
 #pragma unroll
     for (int s = 0; s < 16; ++s) hn[s] = gld2(hbase + size_t(r_begin) * S + tid, size_t(s * NT));
@@ -85,15 +86,15 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
       float2 v[16];
 #pragma unroll
       for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], hn[s]);
-      if constexpr (kSingle)
-        fft.inverse_single(v, la, tid);
-      else
-        fft.inverse(v, la, lb, tid);
       {
         const float2* __restrict__ h = hbase + size_t(min(r + 1, r_end - 1)) * S + tid;  // unconditional: no branch
 #pragma unroll
         for (int s = 0; s < 16; ++s) hn[s] = gld2(h, size_t(s * NT));
       }
+      if constexpr (kSingle)
+        fft.inverse_single(v, la, tid);
+      else
+        fft.inverse(v, la, lb, tid);
       // Slot a of this thread is FOP bin out0 + NT*a + tid - overlap: one base
       // address per row and a constant offset per slot (the store's immediate).
       float* __restrict__ out = job->fop + size_t(r) * n_bins + (ptrdiff_t(out0) + tid - ptrdiff_t(overlap));
