@@ -66,7 +66,6 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
       xf[a] = make_float2(0.f, 0.f);
       if (idx >= 0 && idx < int(n_bins)) xf[a] = gld2(x, size_t(idx));
     }
-    fft.forward(xf, la, tid);
 
     const uint32_t r_begin = job->row_begin + chunk * job->rows_per_item;
     uint32_t r_end = r_begin + job->rows_per_item;
@@ -75,13 +74,15 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
     const float2* __restrict__ hbase = static_cast<const float2*>(job->hperm);
     const uint32_t out0 = tile * payload;  // FOP bin of tile sample s = overlap
 
-    // The template spectrum of the next row is loaded as soon as this row's has
-    // been used, so the loads have a whole inverse transform to complete (and
-    // they precede this row's FOP stores: loads and stores share one in-order
-    // completion counter, so a load issued after the stores would wait for them).
+    // Template spectra are loaded one row ahead: the first row's before the
+    // forward transform, every next row's as soon as this row's has been used,
+    // so the loads have a whole transform to complete (and they precede this
+    // row's FOP stores: loads and stores share one in-order completion counter,
+    // so a load issued after the stores would wait for them).
     float2 hn[16];
 #pragma unroll
     for (int s = 0; s < 16; ++s) hn[s] = gld2(hbase + size_t(r_begin) * S + tid, size_t(s * NT));
+    fft.forward(xf, la, tid);
     for (uint32_t r = r_begin; r < r_end; ++r) {
       float2 v[16];
 #pragma unroll
