@@ -692,19 +692,18 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
   }
   std::vector<uint32_t> rpis = e->cfg.rows_per_item ? std::vector<uint32_t>{e->cfg.rows_per_item}
                                                     : std::vector<uint32_t>{8, 16, 32, 64};
-  std::vector<uint32_t> hrows = e->cfg.hsum_rows ? std::vector<uint32_t>{e->cfg.hsum_rows}
-                                                 : std::vector<uint32_t>{6, kHsumRowsPerBlock, 22};
+  // Rows per summing workgroup are not timed: the timing runs on a zero spectrum,
+  // which has no detections, while on real data shorter chunks bound the tail of
+  // the workgroups that meet dense detections (11 rows measured best there).
   // Defaults used when nothing needs timing.
   out->tile = tiles.size() == 1 ? tiles[0] : (std::find(tiles.begin(), tiles.end(), 2048u) != tiles.end() ? 2048u : tiles[0]);
   out->rows_per_item = rpis.size() == 1 ? rpis[0] : 16;
-  out->hsum_rows = hrows.size() == 1 ? hrows[0] : kHsumRowsPerBlock;
-  const bool tune_ftc = tiles.size() > 1 || rpis.size() > 1, tune_hsum = hrows.size() > 1;
-  if (tune_ftc || tune_hsum) {
+  out->hsum_rows = e->cfg.hsum_rows ? e->cfg.hsum_rows : kHsumRowsPerBlock;
+  if (tiles.size() > 1 || rpis.size() > 1) {
     const uint32_t spill = e->cfg.spill_per_harmonic ? e->cfg.spill_per_harmonic : default_spill(T, n_bins);
     const TrialLayout lay = trial_layout(n_bins, T, 8, e->cfg.cands_per_harmonic, spill, false);
     hipError_t herr = hipSuccess;
     char* block = e->arena->alloc(lay.total, &herr);
-    char* small = block;
     char* mem = block ? block + lay.small_bytes : nullptr;
     if (!block) {
       info(e, "autotune of N=%u bank %u skipped: no room in the HBM budget; using tile %u", n_bins, bank_id,
@@ -715,7 +714,7 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
       float best = FLT_MAX;
       for (uint32_t tile : tiles) {
         for (uint32_t rpi : rpis) {
-          if (st != FDAS_OK || !tune_ftc) break;
+          if (st != FDAS_OK) break;
           uint32_t n_items = 0;
           std::vector<FtcJob> jobs = shape_jobs(bank, tile, n_bins, rb, nr, rpi, &n_items);
           for (FtcJob& j : jobs) {
@@ -742,52 +741,6 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
             best = ms;
             out->tile = tile;
             out->rows_per_item = rpi;
-          }
-        }
-      }
-      if (st == FDAS_OK && tune_hsum) {
-        // The FOP of the last timed (or a fresh) convolution is zero; thresholds
-        // at FLT_MAX keep the lists empty, so only the streaming cost is timed.
-        HsumJob j{};
-        j.fop = reinterpret_cast<const float*>(mem + lay.fop);
-        j.n_bins = n_bins;
-        j.n_templates = T;
-        j.row_begin = rb;
-        j.n_rows = nr;
-        j.n_harmonics = 8;
-        j.spill_cap = spill;
-        j.spill_key = reinterpret_cast<uint32_t*>(mem + lay.spill_key);
-        j.spill_pwr = reinterpret_cast<float*>(mem + lay.spill_pwr);
-        j.counts = reinterpret_cast<uint32_t*>(small + lay.counts);
-        for (uint32_t k = 0; k < kMaxHarmonics; ++k) j.thr[k] = FLT_MAX;
-        if (!tune_ftc) {
-          // The FOP block is uninitialised: clear it so the thresholds hold.
-          st = hipMemsetAsync(mem + lay.fop, 0, 4ull * T * n_bins, e->s_ftc) == hipSuccess
-                   ? FDAS_OK
-                   : fail(e, FDAS_EDEVICE, "FOP clear failed while autotuning");
-        }
-        float bestH = FLT_MAX;
-        for (uint32_t hr : hrows) {
-          if (st != FDAS_OK) break;
-          hsum_row_chunks(nr, &j.rows_per_block, &j.n_row_chunks, hr);
-          const uint32_t blocks = (n_bins + kHsumBinsPerBlock - 1) / kHsumBinsPerBlock * j.n_row_chunks;
-          if (hipMemcpy(g.d_hsum, &j, sizeof(HsumJob), hipMemcpyHostToDevice) != hipSuccess) {
-            st = fail(e, FDAS_EDEVICE, "job upload failed while autotuning");
-            break;
-          }
-          float ms = FLT_MAX;
-          st = time_best(
-              e,
-              [&]() -> fdas_status {
-                EHIP(e, launch_zero_u32(j.counts, 1, e->s_ftc));
-                EHIP(e, launch_hsum(g.d_hsum, 1, blocks, 8, hsum_vec_ok(n_bins), e->s_ftc));
-                return FDAS_OK;
-              },
-              2, &ms);
-          info(e, "autotune N=%u bank %u: %u rows per summing workgroup: %.4f ms", n_bins, bank_id, hr, ms);
-          if (st == FDAS_OK && ms < bestH) {
-            bestH = ms;
-            out->hsum_rows = hr;
           }
         }
       }

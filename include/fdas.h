@@ -261,7 +261,7 @@ typedef struct fdas_plan fdas_plan;
 typedef struct fdas_engine_config {
   uint32_t tile;               /* FFT size for every shape; 0: autotuned per shape from 1024/2048/4096 */
   uint32_t rows_per_item;      /* templates per FT-convolution workgroup; 0: autotuned (8/16/32/64) */
-  uint32_t hsum_rows;          /* rows per harmonic-summing workgroup; 0: autotuned (6/11/22) */
+  uint32_t hsum_rows;          /* rows per harmonic-summing workgroup; 0: 11 */
   uint32_t cands_per_harmonic; /* candidates kept per plane, <= 8192 (default 4096) */
   uint32_t spill_per_harmonic; /* detections buffered per plane before the cap
                                   (0, the default: max(262144, T * N / 512) per trial) */
