@@ -11,21 +11,27 @@ workload: 64 trials of N = 2^21 (seeds 1000..1063, 0-2 drifting tones each),
 spectra resident in HBM before the timed region, one step = one batch of 64
 trials.  Two plans alternate so that one batch's tail overlaps the next
 batch's head; inside a plan, groups of trials pipeline FT convolution on one
-stream against harmonic summing on another.
+stream against harmonic summing on another.  The engine's autotuner fixes
+the tile, rows per item and rows per summing workgroup; the shape is recorded
+in ``config``.
 
 Also reported in the same JSON line:
+  * plain_gpu_form (primary plain form): one trial per launch on the same
+    resident spectra at the same tile: device-to-device input copy, FT
+    convolution, summing, selection, list read-back, stream synchronised per
+    trial; p50 / p99 latency;
+  * plain_gpu_form_h2d: the same with the 16.8 MB host spectrum uploaded per
+    trial;
   * single_trial_form: the drop-in class FDAS path, two trials in flight (A/B);
-  * plain_gpu_form: one trial per launch, host spectrum uploaded, lists read
-    back and the stream synchronised per trial (p50 / p99 latency);
   * graph_form: the same batches replayed as HIP graphs;
   * uneven_form: 48 trials of N in {2^19..2^22}, banks of 85 / 169 templates,
     8 or 16 harmonics, in one plan;
   * served_form: open-loop Poisson request streams through the serving loop,
     p50 / p99 latency and throughput beside the plain one-request-per-launch
-    form on the same arrivals;
+    form on the same arrivals, and the sustained capacity of both;
   * roofline of the dominant kernel (HIP-event spans on its own stream over
     the timed region) and the CPU baseline (the oracle's C restatement timed
-    on this host).
+    on this host: all threads and one thread, at N = 2^21 and N = 32,768).
 """
 
 from __future__ import annotations
@@ -69,12 +75,17 @@ def hsum_bytes(n_bins, n_templates, n_cands, trials=1):
     return trials * (4 * n_templates * n_bins) + 8 * n_cands
 
 
-def fft_flops(n_bins, n_templates, tile, overlap):
-    """Nominal 5 S log2 S per transform, forward per tile + inverse per (tile, template)."""
-    payload = tile - overlap
-    n_tiles = -(-n_bins // payload)
+def ftc_kernel_name(tile):
+    return "ftconv1k_kernel" if tile == 1024 else f"ftconv_kernel<{tile}>"
+
+
+def ftc_flops(shape, tile):
+    """FP32 flops of one trial's FT convolution from the transforms the kernel runs: 5 S log2 S per
+    transform (forward per work item, inverse per tile and template row), 6 S per spectrum product
+    and 3 per stored |x|^2."""
     per = 5.0 * tile * np.log2(tile)
-    return n_tiles * per + n_tiles * n_templates * (per + 6 * tile + 3 * payload)
+    return ((shape["forward_transforms"] + shape["inverse_transforms"]) * per
+            + shape["inverse_transforms"] * 6.0 * tile + shape["stored_outputs"] * 3.0)
 
 
 def device_thresholds(x, n_harmonics, sigma=6.0):
@@ -89,36 +100,54 @@ def pct(a, q):
     return round(float(np.percentile(np.asarray(a), q)), 3) if len(a) else None
 
 
-def cpu_baseline(x, bank, thr, cap, budget_s=10.0):
-    """The oracle's C restatement of ft_conv.py + hsum.py (FP32, OpenMP, own FFT) on full trials.
-
-    Runs whole N = 2^21 trials back to back until ~budget_s of CPU time has passed (at least 2).
-    """
+def _cpu_trial_times(x, bank, thr, cap, threads, budget_s, min_reps, max_reps):
     from oracle import cpu
-    n_threads = cpu.threads()
-    cpu.fop(x[:4096], bank)  # warm-up (page-in, thread pool)
     times = []
     t_start = time.perf_counter()
-    while len(times) < 2 or (time.perf_counter() - t_start < budget_s and len(times) < 50):
+    while len(times) < min_reps or (time.perf_counter() - t_start < budget_s and len(times) < max_reps):
         t0 = time.perf_counter()
-        fop = cpu.fop(x, bank)
-        cpu.hsum(fop, thr, len(thr), cap)
+        fop = cpu.fop(x, bank, threads=threads)
+        cpu.hsum(fop, thr, len(thr), cap, threads=threads)
         times.append(time.perf_counter() - t0)
         del fop
-    med = float(np.median(times))
+    return float(np.median(times)), len(times)
+
+
+def cpu_baseline(x, bank, thr, cap, budget_s=12.0):
+    """The oracle's C restatement of ft_conv.py + hsum.py (FP32, OpenMP, own radix-4 FFT, AVX2/FMA
+    build where the host has them) on whole trials: N = 2^21 on all host threads (the baseline
+    value) and on one thread, and the emulator-size N = 32,768 search on all threads."""
+    from oracle import cpu
+    from fdas_amd import synth
+    n_threads = cpu.threads()
+    cpu.fop(x[:4096], bank)  # warm-up (page-in, thread pool)
+    med, reps = _cpu_trial_times(x, bank, thr, cap, n_threads, budget_s, 3, 50)
+    med1, reps1 = _cpu_trial_times(x, bank, thr, cap, 1, 0.0, 1, 1)
+    w1 = synth.emulator_workload()
+    x1 = w1.spectrum()
+    thr1 = synth.noise_thresholds(x1, 8)
+    _cpu_trial_times(x1, bank, thr1, 64, n_threads, 0.0, 1, 1)
+    med_e, reps_e = _cpu_trial_times(x1, bank, thr1, 64, n_threads, 1.0, 5, 200)
     return {"value": round(1.0 / med, 4), "unit": "DM-trials/s", "cores": n_threads, "kind": "port",
-            "sample": f"{len(times)} full trials (N=2^21, 85 templates x 421 taps, 8 harmonics, 4096/plane) through "
-                      f"oracle/fdas_cpu.c (FP32 overlap-save FFT convolution + harmonic summing + selection), "
-                      f"median {med:.3f} s per trial on {n_threads} OpenMP threads",
-            "ms_per_trial": round(med * 1000.0, 2)}
+            "sample": f"{reps} full trials (N=2^21, 85 templates x 421 taps, 8 harmonics, 4096/plane) through "
+                      f"oracle/fdas_cpu.c ({cpu.isa()} build: FP32 overlap-save radix-4 FFT convolution + "
+                      f"harmonic summing + selection), median {med:.3f} s per trial on {n_threads} OpenMP threads",
+            "ms_per_trial": round(med * 1000.0, 2), "isa": cpu.isa(),
+            "one_thread": {"value": round(1.0 / med1, 4), "ms_per_trial": round(med1 * 1000.0, 1),
+                           "sample": f"{reps1} full N=2^21 trial on 1 thread"},
+            "emulator_size": {"value": round(1.0 / med_e, 2), "ms_per_trial": round(med_e * 1000.0, 3),
+                              "cores": n_threads,
+                              "sample": f"median of {reps_e} N=32,768 trials (85 templates, 8 harmonics, 64/plane)"}}
 
 
 # --------------------------------------------------------------------------- single-trial drop-in path
 
-def single_trial_forms(args, x, bank, thr, forms):
+def single_trial_forms(args, x, bank, thr, forms, tile, device_trials):
+    """The drop-in class FDAS path at the batched form's tile.  device_trials: [(spectrum tensor on the
+    device, thresholds)] of the batched workload, the inputs of the device-resident plain form."""
     from fdas_amd.host import FDAS
     f = FDAS(log=lambda lvl, msg: (log(msg) if lvl >= 1 else None))
-    assert f.initialise_accelerator(N_METRIC, tile=2048, n_templates=T_METRIC, max_taps=421,
+    assert f.initialise_accelerator(N_METRIC, tile=tile, n_templates=T_METRIC, max_taps=421,
                                     n_harmonics=H_METRIC, cands_per_harmonic=CAP), f.last_error
     for ab in (FDAS.A, FDAS.B):
         assert f.upload_template_taps(bank, ab), f.last_error
@@ -153,25 +182,58 @@ def single_trial_forms(args, x, bank, thr, forms):
         ms = (time.perf_counter() - t0) * 1000.0 / 30
         st = np.array(rec)
         out["single_trial_form"] = {
-            "value": round(1000.0 / ms, 2), "unit": "DM-trials/s", "ms_per_trial": round(ms, 4),
+            "value": round(1000.0 / ms, 2), "unit": "DM-trials/s", "ms_per_trial": round(ms, 4), "tile": tile,
             "ftconv_ms": round(float(st[:, 1].mean()), 4), "hsum_ms": round(float(st[:, 2].mean()), 4),
             "select_ms": round(float(st[:, 3].mean()), 4),
             "form": "class FDAS drop-in path, launch_resident on buffer sets A/B, two trials in flight"}
         log(f"single-trial: {out['single_trial_form']}")
     if "plain" in forms:
+        import torch
+
+        def plain_stats(lat, total_s, form):
+            lat = np.array(lat)
+            return {"value": round(float(len(lat) / total_s), 2), "unit": "DM-trials/s",
+                    "ms_per_trial": round(float(lat.mean()), 4), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                    "trials": int(len(lat)), "tile": tile, "form": form}
+
+        # device-resident: the batched workload's spectra, one trial per launch, synchronised per trial
+        n_dev = len(device_trials)
+        for i in range(4):
+            xd, th = device_trials[i % n_dev]
+            assert f.set_input_device(xd.data_ptr(), FDAS.A), f.last_error
+            assert f.launch_resident(th, loc[FDAS.A], pwr[FDAS.A], part, FDAS.A), f.last_error
+            assert f.wait(FDAS.A), f.last_error
+        torch.cuda.synchronize()
         lat = []
-        for i in range(args.plain_trials + 2):
+        t_all = time.perf_counter()
+        for i in range(args.plain_trials):
+            xd, th = device_trials[i % n_dev]
+            t = time.perf_counter()
+            assert f.set_input_device(xd.data_ptr(), FDAS.A), f.last_error
+            assert f.launch_resident(th, loc[FDAS.A], pwr[FDAS.A], part, FDAS.A), f.last_error
+            assert f.wait(FDAS.A), f.last_error
+            lat.append((time.perf_counter() - t) * 1000.0)
+        out["plain_gpu_form"] = plain_stats(
+            lat, time.perf_counter() - t_all,
+            "device-resident spectrum: device-to-device input copy + FT conv + HSUM + selection + list read-back "
+            "+ sync, one trial per launch, the batched workload's trials")
+        log(f"plain (device-resident): {out['plain_gpu_form']}")
+        # host upload per trial (the reference's launch(): host spectrum in, lists out)
+        for _ in range(2):
+            assert f.launch(x, thr, loc[FDAS.A], pwr[FDAS.A], part, FDAS.A), f.last_error
+            assert f.wait(FDAS.A), f.last_error
+        lat = []
+        t_all = time.perf_counter()
+        for _ in range(args.plain_trials):
             t = time.perf_counter()
             assert f.launch(x, thr, loc[FDAS.A], pwr[FDAS.A], part, FDAS.A), f.last_error
             assert f.wait(FDAS.A), f.last_error
-            if i >= 2:
-                lat.append((time.perf_counter() - t) * 1000.0)
-        lat = np.array(lat)
-        out["plain_gpu_form"] = {
-            "value": round(float(1000.0 / lat.mean()), 2), "unit": "DM-trials/s",
-            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-            "form": "host spectrum upload + FT conv + HSUM + selection + read-back + sync, one trial per launch"}
-        log(f"plain: {out['plain_gpu_form']}")
+            lat.append((time.perf_counter() - t) * 1000.0)
+        out["plain_gpu_form_h2d"] = plain_stats(
+            lat, time.perf_counter() - t_all,
+            "host spectrum upload (16.8 MB, pageable) + FT conv + HSUM + selection + read-back + sync, "
+            "one trial per launch")
+        log(f"plain (host upload): {out['plain_gpu_form_h2d']}")
     f.close()
     return out
 
@@ -182,7 +244,7 @@ def make_engine(args, **kw):
     from fdas_amd.engine import Engine
     return Engine(tile=args.tile, rows_per_item=args.rows_per_item, hsum_rows=args.hsum_rows,
                   cands_per_harmonic=CAP, group_trials=args.group_trials,
-                  ftc_workgroups_per_cu=args.ftc_wg_per_cu,
+                  ftc_workgroups_per_cu=args.ftc_wg_per_cu, fusion=not args.no_fusion,
                   log=lambda lvl, msg: (log(msg) if (lvl >= 1 or args.verbose) else None), **kw)
 
 
@@ -204,27 +266,37 @@ def run_batches(plans, n, on_done=None):
             on_done(q)
 
 
-def batched_form(args, eng, bank85):
+def batch_workload(args):
+    """The batched-uniform trials on the device: [(spectrum tensor, thresholds)], seeds 1000.."""
     import torch
     from fdas_amd import synth
-    from fdas_amd.engine import Trial
-    log(f"batched: generating {args.batch} trials of N=2^21 on the device")
-    trials = []
+    log(f"generating {args.batch} trials of N=2^21 on the device")
+    out = []
     for i in range(args.batch):
-        w = synth.random_trial(1000 + i, N_METRIC)
-        x = synth.device_spectrum(w)
-        trials.append(Trial(x, bank85, device_thresholds(x, H_METRIC), 2, tag=1000 + i))
+        x = synth.device_spectrum(synth.random_trial(1000 + i, N_METRIC))
+        out.append((x, device_thresholds(x, H_METRIC)))
     torch.cuda.synchronize()
+    return out
+
+
+def batched_form(args, eng, bank85, workload):
+    from fdas_amd.engine import Trial
+    trials = [Trial(x, bank85, thr, 2, tag=1000 + i) for i, (x, thr) in enumerate(workload)]
     tile = eng.prepare(N_METRIC, bank85, 2)
-    out = {"tile": tile}
+    shape = eng.shape_info(N_METRIC, bank85, 2)
+    log(f"batched: shape {shape}")
+    out = {"tile": tile, "shape": shape}
     def timed(graph):
         """(ms per batch, mean FT-conv span, mean HSUM span per group) over args.steps batches."""
-        spans = {"ftc": [], "hsum": []}
+        spans = {"ftc": [], "hsum": [], "fused": []}
 
         def collect(p):
             info = p.info()
-            spans["ftc"].append(info["ftc_ms"] / info["n_groups"])
-            spans["hsum"].append(info["hsum_ms"] / info["n_groups"])
+            if info["lagged"] and info["n_fused"]:
+                spans["fused"].append(info["fused_ms"] / info["n_fused"])
+            else:
+                spans["ftc"].append(info["ftc_ms"] / info["n_groups"])
+                spans["hsum"].append(info["hsum_ms"] / info["n_groups"])
 
         plans = [eng.plan(trials, graph=graph) for _ in range(2)]
         out["groups_per_batch"] = plans[0].info()["n_groups"]
@@ -240,21 +312,33 @@ def batched_form(args, eng, bank85):
         res = plans[0].wait(raise_overflow=False)
         out["statuses_ok"] = all(r.status == 0 for r in res)
         out["candidates_per_trial_mean"] = float(np.mean([int(np.sum(r.n_kept)) for r in res]))
+        # Batch latency: one batch in flight, from launch to every list in host memory.
+        lat = []
+        for _ in range(max(5, args.steps)):
+            t0 = time.perf_counter()
+            plans[0].launch()
+            plans[0].wait(results=False)
+            lat.append((time.perf_counter() - t0) * 1000.0)
+        out["lagged"] = bool(plans[0].info()["lagged"])
         for p in plans:
             p.close()
-        return ms, float(np.mean(spans["ftc"])), float(np.mean(spans["hsum"]))
+        mean = lambda v: float(np.mean(v)) if v else 0.0
+        return ms, {k: mean(v) for k, v in spans.items()}, lat
 
     log(f"batched: timing {args.steps} batches of {args.batch} (eager launches)")
-    ms, ftc, hsum = timed(False)
+    ms, sp, lat = timed(False)
     out["eager"] = {"value": round(args.batch * 1000.0 / ms, 2), "ms_per_step": round(ms, 4),
-                    "ftc_span_ms": round(ftc, 4), "hsum_span_ms": round(hsum, 4)}
+                    "ftc_span_ms": round(sp["ftc"], 4), "hsum_span_ms": round(sp["hsum"], 4),
+                    "fused_span_ms": round(sp["fused"], 4),
+                    "batch_latency_p50_ms": pct(lat, 50), "batch_latency_p99_ms": pct(lat, 99)}
     log(f"batched eager: {out['eager']}")
     if not args.no_graph:
         # The same launches captured once per plan and replayed; kernel spans are
         # not recorded inside the graph, so the headline stays the eager form.
         log(f"batched: timing {args.steps} batches of {args.batch} (HIP graph replays)")
-        gms, _, _ = timed(True)
-        out["graph"] = {"value": round(args.batch * 1000.0 / gms, 2), "ms_per_step": round(gms, 4)}
+        gms, _, glat = timed(True)
+        out["graph"] = {"value": round(args.batch * 1000.0 / gms, 2), "ms_per_step": round(gms, 4),
+                        "batch_latency_p50_ms": pct(glat, 50), "batch_latency_p99_ms": pct(glat, 99)}
         log(f"batched graph: {out['graph']}")
     form = "eager"
     sel = out[form]
@@ -263,7 +347,9 @@ def batched_form(args, eng, bank85):
     out["value"] = args.batch * 1000.0 / sel["ms_per_step"]
     out["ftc_span_ms"] = sel["ftc_span_ms"]
     out["hsum_span_ms"] = sel["hsum_span_ms"]
-    del trials
+    out["fused_span_ms"] = sel["fused_span_ms"]
+    out["batch_latency_p50_ms"] = sel["batch_latency_p50_ms"]
+    out["batch_latency_p99_ms"] = sel["batch_latency_p99_ms"]
     return out
 
 
@@ -323,14 +409,15 @@ def served_form(args, eng, banks, bank_taps):
                 x = synth.device_spectrum(w)
                 pool.setdefault((n, T), []).append((x, device_thresholds(x, H_METRIC)))
     torch.cuda.synchronize()
+    tiles = {}
     for (n, T) in pool:
-        eng.prepare(n, banks[T], 2)
+        tiles[(n, T)] = eng.prepare(n, banks[T], 2)
 
-    # plain one-request-per-launch server: one class FDAS context per shape
+    # plain one-request-per-launch server: one class FDAS context per shape, at the engine's tile
     plain_ctx = {}
     for (n, T) in pool:
         f = FDAS(log=lambda lvl, msg: (log(msg) if lvl >= 1 else None))
-        assert f.initialise_accelerator(n, n_templates=T, max_taps=421, tile=2048, n_harmonics=H_METRIC,
+        assert f.initialise_accelerator(n, n_templates=T, max_taps=421, tile=tiles[(n, T)], n_harmonics=H_METRIC,
                                         cands_per_harmonic=CAP), f.last_error
         assert f.upload_template_taps(bank_taps[T], FDAS.A), f.last_error
         plain_ctx[(n, T)] = f
@@ -345,28 +432,25 @@ def served_form(args, eng, banks, bank_taps):
     # warm every path once
     for key, lst in pool.items():
         plain_one(key, *lst[0])
-    rates = [float(r) for r in args.serve_rates.split(",") if r]
-    out = {"rates": {}, "workload": "Poisson arrivals; N in {2^20,2^21,2^22} (0.3/0.5/0.2), banks 85/169 x 421 taps, "
-                                    "H = 8, 4096 candidates/plane, spectra resident in HBM",
-           "hbm_budget_gb": 200}
-    for rate in rates:
-        rng = np.random.default_rng(int(rate) + 17)
-        n_req = max(40, int(rate * args.serve_seconds))
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+
+    def picks_for(n_req, seed):
+        rng = np.random.default_rng(seed)
         picks = []
-        for i in range(n_req):
+        for _ in range(n_req):
             n = int(rng.choice(sizes, p=weights))
             T = int(rng.choice([85, 169]))
             x, thr = pool[(n, T)][int(rng.integers(0, 3))]
             picks.append(((n, T), x, thr))
-        # served form
+        return picks
+
+    def serve(picks, arrivals):
+        """Served form on the given arrival times (s from the start): (throughput, latencies, statuses ok)."""
+        n_req = len(picks)
         results = []
-        done_evt = threading.Event()
 
         def poller():
             while len(results) < n_req:
                 results.extend(eng.poll(max_results=64, timeout_ms=200))
-            done_evt.set()
 
         th = threading.Thread(target=poller, daemon=True)
         th.start()
@@ -376,14 +460,14 @@ def served_form(args, eng, banks, bank_taps):
             if delay > 0:
                 time.sleep(delay)
             eng.submit(Trial(x, banks[key[1]], thr, 2, tag=i))
-        th.join(timeout=120)
+        th.join(timeout=300)
         t_end = time.perf_counter()
         assert len(results) == n_req, f"served {len(results)} of {n_req}"
-        lat = [r.latency_ms for r in results]
-        served = {"requests": n_req, "throughput": round(n_req / (t_end - t_start), 2), "p50_ms": pct(lat, 50),
-                  "p99_ms": pct(lat, 99), "statuses_ok": all(r.status == 0 for r in results)}
-        # plain form on the same arrivals: FIFO, one request per launch
-        plat = []
+        return n_req / (t_end - t_start), [r.latency_ms for r in results], all(r.status == 0 for r in results)
+
+    def plain(picks, arrivals):
+        """Plain form on the same arrivals: FIFO, one request per launch."""
+        lat = []
         t_start = time.perf_counter()
         for i, (key, x, thr) in enumerate(picks):
             arr = t_start + arrivals[i]
@@ -391,11 +475,42 @@ def served_form(args, eng, banks, bank_taps):
             if delay > 0:
                 time.sleep(delay)
             plain_one(key, x, thr)
-            plat.append((time.perf_counter() - arr) * 1000.0)
-        t_end = time.perf_counter()
-        plain = {"throughput": round(n_req / (t_end - t_start), 2), "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99)}
-        out["rates"][str(int(rate))] = {"served": served, "plain": plain}
-        log(f"serve {rate:.0f} req/s: served {served} | plain {plain}")
+            lat.append((time.perf_counter() - arr) * 1000.0)
+        return len(picks) / (time.perf_counter() - t_start), lat
+
+    def at_rate(rate, seed):
+        n_req = max(40, int(rate * args.serve_seconds))
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        picks = picks_for(n_req, seed + 1)
+        thr_s, lat_s, ok = serve(picks, arrivals)
+        thr_p, lat_p = plain(picks, arrivals)
+        served = {"requests": n_req, "throughput": round(thr_s, 2), "p50_ms": pct(lat_s, 50),
+                  "p99_ms": pct(lat_s, 99), "statuses_ok": ok}
+        plain_r = {"throughput": round(thr_p, 2), "p50_ms": pct(lat_p, 50), "p99_ms": pct(lat_p, 99)}
+        return {"offered": round(rate, 1), "served": served, "plain": plain_r}
+
+    out = {"rates": {}, "workload": "Poisson arrivals; N in {2^20,2^21,2^22} (0.3/0.5/0.2), banks 85/169 x 421 taps, "
+                                    "H = 8, 4096 candidates/plane, spectra resident in HBM; plain form at the "
+                                    "engine's tile per shape",
+           "hbm_budget_gb": 200, "tiles": {f"N={n},T={T}": t for (n, T), t in tiles.items()}}
+    for rate in [float(r) for r in args.serve_rates.split(",") if r]:
+        out["rates"][str(int(rate))] = r = at_rate(rate, int(rate) + 17)
+        log(f"serve {rate:.0f} req/s: {r}")
+    # Capacity: every request offered at once, served and plain, on the same requests.
+    picks = picks_for(args.capacity_requests, 7)
+    zero = np.zeros(len(picks))
+    cap_s, lat_s, ok = serve(picks, zero)
+    cap_p, _ = plain(picks, zero)
+    out["capacity"] = {"requests": len(picks), "served_req_per_s": round(cap_s, 1),
+                       "plain_req_per_s": round(cap_p, 1), "served_vs_plain": round(cap_s / cap_p, 2),
+                       "statuses_ok": ok}
+    log(f"serve capacity: {out['capacity']}")
+    # Latency at 50 %, 80 % and 100 % of the served capacity (plain on the same arrivals).
+    out["load"] = {}
+    for frac in (0.5, 0.8, 1.0):
+        out["load"][f"{int(frac * 100)}%"] = r = at_rate(frac * cap_s, 100 + int(frac * 100))
+        log(f"serve at {int(frac * 100)} % of capacity: {r}")
     st = eng.stats()
     out["engine_stats"] = {k: int(v) for k, v in st.items()}
     for f in plain_ctx.values():
@@ -420,8 +535,11 @@ def main():
     ap.add_argument("--ftc-wg-per-cu", type=int, default=0,
                     help="FT-convolution workgroups resident per CU (0: one per work item)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="plans launch FT convolution and harmonic summing as separate kernels per group")
     ap.add_argument("--plain-trials", type=int, default=20)
-    ap.add_argument("--serve-rates", default="50,100,200,400,800")
+    ap.add_argument("--serve-rates", default="50,100,200,400")
+    ap.add_argument("--capacity-requests", type=int, default=400)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
@@ -444,32 +562,40 @@ def main():
     bank169 = fdas_amd.make_drift_bank(synth.drift_grid(169), 421)
     thr = synth.noise_thresholds(x, H_METRIC)
 
-    out_forms = single_trial_forms(args, x, bank85, thr, forms) if forms & {"single", "plain"} else {}
-
     batched = uneven = served = None
-    eng = None
-    if forms & {"batched", "uneven", "serve"}:
-        eng = make_engine(args)
-        banks = {85: eng.add_bank(bank85), 169: eng.add_bank(bank169)}
-        if "batched" in forms:
-            batched = batched_form(args, eng, banks[85])
-        if "uneven" in forms:
-            uneven = uneven_form(args, eng, banks)
-        if "serve" in forms:
-            served = served_form(args, eng, banks, {85: bank85, 169: bank169})
-        eng.close()
+    eng = make_engine(args)
+    banks = {85: eng.add_bank(bank85), 169: eng.add_bank(bank169)}
+    tile = eng.prepare(N_METRIC, banks[85], 2)
+    shape = eng.shape_info(N_METRIC, banks[85], 2)
+    workload_trials = batch_workload(args) if forms & {"batched", "plain"} else []
+    if "batched" in forms:
+        batched = batched_form(args, eng, banks[85], workload_trials)
+    out_forms = single_trial_forms(args, x, bank85, thr, forms, tile, workload_trials) \
+        if forms & {"single", "plain"} else {}
+    if "uneven" in forms:
+        uneven = uneven_form(args, eng, banks)
+    if "serve" in forms:
+        served = served_form(args, eng, banks, {85: bank85, 169: bank169})
+    eng.close()
+    del workload_trials
 
     if batched is not None:
         value, ms_per_step = batched["value"], batched["ms_per_step"]
-        steps, tile, trials_per_launch = args.steps, batched["tile"], args.group_trials
-        kernels = {
-            f"ftconv_kernel<{tile}>": {"ms": batched["ftc_span_ms"],
-                                       "bytes": ftc_bytes(N_METRIC, T_METRIC, tile, trials_per_launch)},
-            "hsum_kernel<8,true>": {"ms": batched["hsum_span_ms"],
-                                    "bytes": hsum_bytes(N_METRIC, T_METRIC,
-                                                        trials_per_launch * batched["candidates_per_trial_mean"],
-                                                        trials_per_launch)},
-        }
+        steps, trials_per_launch = args.steps, args.group_trials
+        n_cands = trials_per_launch * batched["candidates_per_trial_mean"]
+        if batched["fused_span_ms"] > 0:
+            # one launch: FT convolution of one group + harmonic summing of the group before
+            kernels = {"fused1k_kernel<8,true>": {
+                "ms": batched["fused_span_ms"],
+                "bytes": ftc_bytes(N_METRIC, T_METRIC, tile, trials_per_launch)
+                + hsum_bytes(N_METRIC, T_METRIC, n_cands, trials_per_launch)}}
+        else:
+            kernels = {
+                ftc_kernel_name(tile): {"ms": batched["ftc_span_ms"],
+                                        "bytes": ftc_bytes(N_METRIC, T_METRIC, tile, trials_per_launch)},
+                "hsum_kernel<8,true>": {"ms": batched["hsum_span_ms"],
+                                        "bytes": hsum_bytes(N_METRIC, T_METRIC, n_cands, trials_per_launch)},
+            }
         workload = (f"batched uniform: {args.batch} DM trials per step, N=2^21 bins, 85 templates x 421 taps, "
                     f"8 harmonics, 4096 candidates/harmonic, spectra resident, groups of {args.group_trials}, "
                     f"{batched['form']} launches")
@@ -478,8 +604,8 @@ def main():
         st = out_forms.get("single_trial_form")
         if st is None:
             raise SystemExit("bench.py: --forms needs batched or single for the headline value")
-        value, ms_per_step, steps, tile, trials_per_launch = st["value"], st["ms_per_trial"], 30, 2048, 1
-        kernels = {"ftconv_kernel<2048>": {"ms": st["ftconv_ms"], "bytes": ftc_bytes(N_METRIC, T_METRIC, 2048)},
+        value, ms_per_step, steps, trials_per_launch = st["value"], st["ms_per_trial"], 30, 1
+        kernels = {ftc_kernel_name(tile): {"ms": st["ftconv_ms"], "bytes": ftc_bytes(N_METRIC, T_METRIC, tile)},
                    "hsum_kernel<8,true>": {"ms": st["hsum_ms"], "bytes": hsum_bytes(N_METRIC, T_METRIC, 8 * CAP)}}
         workload = "single-trial FDAS, N=2^21 bins, 85 templates x 421 taps, 8 harmonics, spectra resident"
         in_flight = "2 trials (buffer sets A/B)"
@@ -487,7 +613,7 @@ def main():
     dominant = max(kernels, key=lambda k: kernels[k]["ms"])
     kd = kernels[dominant]
     achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9
-    ftc_name = [k for k in kernels if k.startswith("ftconv")][0]
+    ftc_name = [k for k in kernels if k.startswith("ftconv") or k.startswith("fused")][0]
     traffic = MEASURED_TRAFFIC.get((dominant, trials_per_launch)) if batched is not None else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(traffic) if traffic else None,
@@ -497,8 +623,9 @@ def main():
                 "trials_per_launch": trials_per_launch,
                 "kernels": {k: {"ms": round(v["ms"], 4), "algorithmic_bytes": int(v["bytes"]),
                                 "gbs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in kernels.items()},
-                "ftconv_fft_tflops": round(trials_per_launch * fft_flops(N_METRIC, T_METRIC, tile, 420)
+                "ftconv_fft_tflops": round(trials_per_launch * ftc_flops(shape, tile)
                                            / (kernels[ftc_name]["ms"] * 1e-3) / 1e12, 2),
+                "ftconv_flops_per_trial": int(ftc_flops(shape, tile)),
                 "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS}
 
     cpu = None
@@ -508,6 +635,7 @@ def main():
         log(f"cpu: {cpu}")
 
     plain = out_forms.get("plain_gpu_form")
+    plain_h2d = out_forms.get("plain_gpu_form_h2d")
     out = {
         "metric": "DM-trials/sec (2^21 bins, 85 tmpl, 8 harm)",
         "value": round(value, 2),
@@ -523,12 +651,15 @@ def main():
         "data": "synthetic (seeded SplitMix64 N(0,1) series + 0-2 drifting 8-harmonic tones per trial; drift banks "
                 "z=2(j-c) of 85/169 templates)",
         "config": {"workload": workload, "n_bins": N_METRIC, "n_templates": T_METRIC, "tile": tile,
-                   "n_harmonics": H_METRIC, "cands_per_harmonic": CAP, "batch": args.batch if batched else 1,
-                   "in_flight": in_flight},
+                   "rows_per_item": shape["rows_per_item"], "hsum_rows": shape["hsum_rows"],
+                   "row_groups": shape["n_row_groups"], "n_harmonics": H_METRIC, "cands_per_harmonic": CAP,
+                   "batch": args.batch if batched else 1, "in_flight": in_flight},
         "vs_plain": round(value / plain["value"], 2) if plain else None,
+        "vs_plain_h2d": round(value / plain_h2d["value"], 2) if plain_h2d else None,
         "batched_form": batched,
         "single_trial_form": out_forms.get("single_trial_form"),
         "plain_gpu_form": plain,
+        "plain_gpu_form_h2d": plain_h2d,
         "uneven_form": uneven,
         "served_form": served,
         "roofline": roofline,

@@ -58,7 +58,7 @@ class EngineConfig(C.Structure):
         ("device", C.c_int32),
         ("hbm_budget_bytes", C.c_uint64),
         ("ftc_workgroups_per_cu", C.c_uint32),
-        ("pad_", C.c_uint32),
+        ("no_fusion", C.c_uint32),
     ]
 
 
@@ -91,6 +91,18 @@ class Result(C.Structure):
     ]
 
 
+class ShapeInfo(C.Structure):
+    _fields_ = [
+        ("tile", C.c_uint32),
+        ("rows_per_item", C.c_uint32),
+        ("hsum_rows", C.c_uint32),
+        ("n_row_groups", C.c_uint32),
+        ("forward_transforms", C.c_uint64),
+        ("inverse_transforms", C.c_uint64),
+        ("stored_outputs", C.c_uint64),
+    ]
+
+
 class EngineStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_uint64),
@@ -116,6 +128,9 @@ class PlanInfo(C.Structure):
         ("last_ms", C.c_float),
         ("ftc_ms", C.c_float),
         ("hsum_ms", C.c_float),
+        ("lagged", C.c_uint32),
+        ("n_fused", C.c_uint32),
+        ("fused_ms", C.c_float),
     ]
 
 
@@ -174,6 +189,8 @@ SIGNATURES = {
     "fdas_engine_last_error": (C.c_char_p, [_P]),
     "fdas_engine_add_bank": (_I, [_P, _F, _u32, _u32, _U]),
     "fdas_engine_prepare": (_I, [_P, _u32, _u32, _u32, _U]),
+    "fdas_engine_tune_cache": (_I, [_P, C.c_char_p]),
+    "fdas_engine_shape_info": (_I, [_P, _u32, _u32, _u32, C.POINTER(ShapeInfo)]),
     "fdas_plan_create": (_I, [_P, C.POINTER(Request), _u32, _u32, C.POINTER(_P)]),
     "fdas_plan_set_request": (_I, [_P, _u32, _P, _F]),
     "fdas_plan_launch": (_I, [_P]),

@@ -55,22 +55,6 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
-COPY_PROBE_SRC = CSRC / "copy_probe.hip"
-COPY_PROBE = LIB_DIR / "copy_probe"
-
-
-def build_copy_probe(verbose: bool = False) -> Path:
-    """The counter-calibration copy kernel (fdas_amd/lib/copy_probe)."""
-    LIB_DIR.mkdir(parents=True, exist_ok=True)
-    cmd = [hipcc(), *HIP_FLAGS, "-o", str(COPY_PROBE), str(COPY_PROBE_SRC)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr}")
-    return COPY_PROBE
-
-
 FLOW_TEST_SRC = ROOT / "tests" / "cpp" / "fdas_flow_test.cpp"
 FLOW_TEST = LIB_DIR / "fdas_flow_test"
 
@@ -91,8 +75,7 @@ def build_flow_test(verbose: bool = False) -> Path:
 def build_all(verbose: bool = False):
     lib = build_library(verbose)
     test = build_flow_test(verbose)
-    probe = build_copy_probe(verbose)
-    return lib, test, probe
+    return lib, test
 
 
 if __name__ == "__main__":

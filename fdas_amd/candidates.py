@@ -117,13 +117,13 @@ def format_table(loc, pwr, n_templates=85, frequencies=None):
 
 
 def main(argv=None):
-    ap = argparse.ArgumentParser(description="Prints pulsar candidate lists.")
-    ap.add_argument("locations", metavar="location-file", help="NumPy array containing the detection locations")
-    ap.add_argument("power", metavar="power-file", help="NumPy array containing the detection powers")
+    ap = argparse.ArgumentParser(description="Decode 32-bit candidate lists and print them as a table.")
+    ap.add_argument("locations", metavar="location-file", help=".npy file of uint32 encoded locations (harmonic, template, bin)")
+    ap.add_argument("power", metavar="power-file", help=".npy file of float32 powers, one per location")
     ap.add_argument("-F", "--resolve-frequencies", dest="frequencies", metavar="frequencies-file",
-                    help="resolve the bin numbers to frequencies, using the specified file")
+                    help=".npy table mapping bin numbers to frequencies, printed beside each bin")
     ap.add_argument("-N", "--num-templates", dest="n_tmpl", metavar="n", type=int, default=85,
-                    help="set the number of FDAS templates")
+                    help="templates in the bank the lists came from (default 85)")
     args = ap.parse_args(argv)
     try:
         loc, pwr = load_lists(args.locations, args.power)

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdint>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -69,6 +70,9 @@ class Arena {
     bytes = align_up(std::max<uint64_t>(bytes, 1));
     for (size_t ci = 0; ci < chunks_.size(); ++ci)
       if (char* p = take(ci, bytes)) return p;
+    // Chunks reserved for earlier, smaller requests may together hold the budget
+    // while each is too small: give back the wholly free ones and retry.
+    if (reserved_ + bytes > budget_) release_free_chunks();
     if (reserved_ + bytes > budget_) return nullptr;
     const uint64_t size = std::min(budget_ - reserved_, std::max(bytes, kArenaChunk));
     void* base = nullptr;
@@ -105,6 +109,26 @@ class Arena {
     c.free.emplace(off, size);
   }
 
+  // Returns every chunk that holds no live allocation to the device.
+  void release_free_chunks() {
+    std::vector<size_t> keep_index(chunks_.size(), SIZE_MAX);
+    std::vector<Chunk> kept;
+    for (size_t ci = 0; ci < chunks_.size(); ++ci) {
+      Chunk& c = chunks_[ci];
+      const bool empty = c.free.size() == 1 && c.free.begin()->first == 0 && c.free.begin()->second == c.size;
+      if (empty) {
+        (void)hipFree(c.base);
+        reserved_ -= c.size;
+      } else {
+        keep_index[ci] = kept.size();
+        kept.push_back(std::move(c));
+      }
+    }
+    for (auto& kv : live_) kv.second.first = keep_index[kv.second.first];
+    chunks_ = std::move(kept);
+  }
+
+  uint64_t budget() const { return budget_; }
   uint64_t reserved() const { return reserved_; }
   uint64_t in_use() const { return in_use_; }
   uint64_t peak() const { return peak_; }
@@ -233,6 +257,7 @@ struct GroupBuf {
   hipEvent_t ev_ftc0 = nullptr, ev_ftc1 = nullptr, ev_hsum0 = nullptr, ev_hsum1 = nullptr, ev_done = nullptr;
   hipEvent_t ev_ftc_t1 = nullptr;
   hipEvent_t ev_hsum_done = nullptr;  // harmonic summing finished (the next group's convolution waits on it)
+  bool launch_fused = false;           // lagged schedule: this group's launch also summed the group before
   struct FtcLaunch {
     uint32_t tile, first, n_jobs, n_items;
   };
@@ -263,9 +288,11 @@ struct fdas_engine {
   std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<Shape>> shapes;
   std::map<uint32_t, std::pair<float2*, float2*>> twiddles;  // tile -> (tw1, tw2)
   hipStream_t s_ftc = nullptr, s_hsum = nullptr;
-  float2* zero_input = nullptr;  // 2^22 zero bins (tuning input)
+  float2* tune_input = nullptr;  // 2^22 bins of seeded complex noise, E|X|^2 = 1 (tuning input)
   uint32_t n_cu = 0;
-  uint32_t zero_input_bins = 0;
+  std::string device_key;        // device name, architecture and CU count (tuning-cache key)
+  std::string tune_cache_path;   // tuned shapes are read from and appended to this file
+  std::map<std::string, std::tuple<uint32_t, uint32_t, uint32_t>> tune_cache;  // key -> (tile, rows/item, hsum rows)
 
   // serving state (guarded by q_mu)
   std::mutex q_mu;
@@ -284,6 +311,7 @@ struct fdas_engine {
   };
   std::vector<Ctx> ctxs;
   uint64_t launch_seq = 0;
+  uint32_t live_plans = 0;         // plans holding arena memory (guarded by api_mu)
   hipEvent_t last_hsum = nullptr;  // summing-done event of the last served group
   fdas_engine_stats stats{};
 };
@@ -302,6 +330,7 @@ struct fdas_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = false, launched = false, eager_timed = false;
+  bool fused = false;  // lagged schedule: launch g runs group g's convolution with group g-1's summing
   uint32_t n_launches = 0;
 };
 
@@ -673,10 +702,206 @@ std::vector<FtcJob> shape_jobs(const Bank& bank, uint32_t tile, uint32_t n_bins,
   return jobs;
 }
 
-// Chooses (tile, rows per item, rows per summing workgroup) for a shape by timing
-// the kernels on a zero spectrum; caller holds api_mu and s_ftc is idle or may
-// be waited on.
-fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32_t part, Shape* out) {
+// Seeded complex Gaussian noise, E|X|^2 = 1: SplitMix64 -> Box-Muller (FP64),
+// the same for every engine, so tuning sees the detection density of
+// noise-dominated trials at the usual thresholds.
+std::vector<float2> tuning_noise(uint32_t n) {
+  std::vector<float2> x(n);
+  uint64_t state = 0x5eedf00dull;
+  auto next = [&]() {
+    uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  for (uint32_t i = 0; i < n; ++i) {
+    const double u1 = (double(next() >> 11) + 1.0) * 0x1.0p-53, u2 = double(next() >> 11) * 0x1.0p-53;
+    const double r = std::sqrt(-std::log(u1)), a = 2.0 * 3.14159265358979323846 * u2;  // variance 1/2 per part
+    x[i] = make_float2(float(r * std::cos(a)), float(r * std::sin(a)));
+  }
+  return x;
+}
+
+fdas_status ensure_tune_input(fdas_engine* e) {
+  if (e->tune_input) return FDAS_OK;
+  const std::vector<float2> x = tuning_noise(kMaxBins);
+  fdas_status st = dev_alloc(e, &e->tune_input, kMaxBins);
+  if (st != FDAS_OK) return st;
+  EHIP(e, hipMemcpy(e->tune_input, x.data(), sizeof(float2) * x.size(), hipMemcpyHostToDevice));
+  return FDAS_OK;
+}
+
+// FNV-1a over the bank's taps and sizes.
+uint64_t bank_digest(const Bank& b) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ull;
+  };
+  mix(&b.n_templates, sizeof(b.n_templates));
+  mix(&b.n_taps, sizeof(b.n_taps));
+  mix(b.taps.data(), b.taps.size() * sizeof(float));
+  return h;
+}
+
+// Cache key of a shape: device, N, bank contents, part and the per-plane cap.
+std::string tune_key(const fdas_engine* e, uint32_t n_bins, const Bank& bank, uint32_t part) {
+  char buf[320];
+  std::snprintf(buf, sizeof(buf), "%s|N=%u|bank=%016llx|part=%u|cap=%u", e->device_key.c_str(), n_bins,
+                (unsigned long long)bank_digest(bank), part, e->cfg.cands_per_harmonic);
+  return buf;
+}
+
+// Deterministic shape without timing: the usable tile with the fewest
+// transform points per FOP output over the bank's row groups (S log2 S per
+// transform), 16 rows per item, the default summing rows.
+void model_shape(const fdas_engine* e, const Bank& bank, uint32_t n_bins, uint32_t rb, uint32_t nr,
+                 const std::vector<uint32_t>& tiles, Shape* out) {
+  double best = 0.0;
+  for (uint32_t tile : tiles) {
+    const std::vector<FtcJob> jobs =
+        plan_ftc_rows(tile, n_bins, bank.n_taps, bank.extent, rb, nr, kMaxRowGroups, 16);
+    double cost = 0.0;
+    for (const FtcJob& j : jobs) cost += double(j.n_rows) * j.n_tiles * tile * std::log2(double(tile));
+    if (best == 0.0 || cost < best) {
+      best = cost;
+      out->tile = tile;
+    }
+  }
+  out->rows_per_item = e->cfg.rows_per_item ? e->cfg.rows_per_item : 16;
+  out->hsum_rows = e->cfg.hsum_rows ? e->cfg.hsum_rows : kHsumRowsPerBlock;
+}
+
+// Times FT convolution (tile x rows per item) and then harmonic summing +
+// selection (rows per summing workgroup) for a shape, on the seeded noise
+// spectrum with thresholds (k + 6 sqrt(k)) E|X|^2, and keeps the fastest.
+// Caller holds api_mu; s_ftc is idle or may be waited on.
+fdas_status time_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32_t part,
+                       const std::vector<uint32_t>& tiles, const std::vector<uint32_t>& rpis,
+                       const std::vector<uint32_t>& hrows, Shape* out) {
+  const Bank& bank = *e->banks[bank_id];
+  const uint32_t T = bank.n_templates, H = 8;
+  fdas_status st = ensure_tune_input(e);
+  if (st != FDAS_OK) return st;
+  const uint32_t spill = e->cfg.spill_per_harmonic ? e->cfg.spill_per_harmonic : default_spill(T, n_bins);
+  const TrialLayout lay = trial_layout(n_bins, T, H, e->cfg.cands_per_harmonic, spill, false);
+  hipError_t herr = hipSuccess;
+  char* block = e->arena->alloc(lay.total, &herr);
+  if (!block) {
+    info(e, "autotune of N=%u bank %u skipped: no room in the HBM budget; using tile %u", n_bins, bank_id, out->tile);
+    return FDAS_OK;
+  }
+  char* mem = block + lay.small_bytes;
+  GroupBuf g;
+  st = group_reserve(e, g, kMaxRowGroups, 1, H);
+  auto run_ftc = [&](uint32_t tile, const std::vector<FtcJob>& jobs, uint32_t n_items) -> fdas_status {
+    const auto& tw = e->twiddles.at(tile);
+    EHIP(e, launch_ftconv(int(tile), g.d_ftc, uint32_t(jobs.size()), n_items, tw.first, tw.second,
+                          ftc_grid(e, n_items), e->s_ftc));
+    return FDAS_OK;
+  };
+  auto ftc_jobs = [&](uint32_t tile, uint32_t rpi, uint32_t* n_items) {
+    std::vector<FtcJob> jobs = shape_jobs(bank, tile, n_bins, out->row_begin, out->n_rows, rpi, n_items);
+    for (FtcJob& j : jobs) {
+      j.spectrum = e->tune_input;
+      j.fop = reinterpret_cast<float*>(mem + lay.fop);
+      j.hperm = bank.spectra.at(tile);
+    }
+    return jobs;
+  };
+  float best = FLT_MAX;
+  for (uint32_t tile : tiles) {
+    for (uint32_t rpi : rpis) {
+      if (st != FDAS_OK) break;
+      uint32_t n_items = 0;
+      const std::vector<FtcJob> jobs = ftc_jobs(tile, rpi, &n_items);
+      if (hipMemcpy(g.d_ftc, jobs.data(), sizeof(FtcJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        st = fail(e, FDAS_EDEVICE, "job upload failed while autotuning");
+        break;
+      }
+      float ms = FLT_MAX;
+      st = time_best(e, [&]() { return run_ftc(tile, jobs, n_items); }, 2, &ms);
+      info(e, "autotune N=%u bank %u: tile %u, %u rows/item: %.4f ms", n_bins, bank_id, tile, rpi, ms);
+      if (st == FDAS_OK && ms < best) {
+        best = ms;
+        out->tile = tile;
+        out->rows_per_item = rpi;
+      }
+    }
+  }
+  if (st == FDAS_OK && hrows.size() > 1) {
+    // The winner's FOP, then the summing shapes on it.
+    uint32_t n_items = 0;
+    const std::vector<FtcJob> jobs = ftc_jobs(out->tile, out->rows_per_item, &n_items);
+    if (hipMemcpy(g.d_ftc, jobs.data(), sizeof(FtcJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess)
+      st = fail(e, FDAS_EDEVICE, "job upload failed while autotuning");
+    if (st == FDAS_OK) st = run_ftc(out->tile, jobs, n_items);
+    Trial t;
+    t.req.n_bins = n_bins;
+    t.req.bank = bank_id;
+    t.req.n_harmonics = H;
+    t.req.part = part;
+    for (uint32_t k = 1; k <= H; ++k) t.req.thresholds[k - 1] = float(k + 6.0 * std::sqrt(double(k)));
+    t.bank = &bank;
+    t.lay = lay;
+    t.spill = spill;
+    t.small = block;
+    t.mem = mem;
+    t.input = e->tune_input;
+    float hbest = FLT_MAX;
+    uint32_t chosen = out->hsum_rows;
+    Shape trial_shape;
+    for (uint32_t rows : hrows) {
+      if (st != FDAS_OK) break;
+      trial_shape.row_begin = out->row_begin;
+      trial_shape.n_rows = out->n_rows;
+      trial_shape.tile = out->tile;
+      trial_shape.hsum_rows = rows;
+      t.shape = &trial_shape;
+      const std::vector<Trial*> one{&t};
+      if ((st = build_group(e, g, one)) != FDAS_OK) break;
+      if ((st = upload_group(e, g, 0, group_hsum_jobs(g), e->s_ftc)) != FDAS_OK) break;
+      float ms = FLT_MAX;
+      st = time_best(
+          e,
+          [&]() -> fdas_status {
+            EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(t.small), uint32_t(lay.small_bytes / 4), e->s_ftc));
+            for (const auto& L : g.hsum)
+              EHIP(e, launch_hsum(g.d_hsum + L.first, L.n_jobs, L.n_blocks, L.H, L.vec, e->s_ftc));
+            EHIP(e, launch_select(g.d_sel, g.n_sel, e->s_ftc));
+            return FDAS_OK;
+          },
+          2, &ms);
+      info(e, "autotune N=%u bank %u: %u rows/summing workgroup: %.4f ms", n_bins, bank_id, rows, ms);
+      if (st == FDAS_OK && ms < hbest) {
+        hbest = ms;
+        chosen = rows;
+      }
+    }
+    out->hsum_rows = chosen;
+  }
+  if (hipStreamSynchronize(e->s_ftc) != hipSuccess && st == FDAS_OK)
+    st = fail(e, FDAS_EDEVICE, "HIP call failed while autotuning");
+  free_group(g);
+  e->arena->free(block);
+  return st;
+}
+
+void save_tuned(fdas_engine* e, const std::string& key, const Shape& sh) {
+  e->tune_cache[key] = std::make_tuple(sh.tile, sh.rows_per_item, sh.hsum_rows);
+  if (e->tune_cache_path.empty()) return;
+  if (FILE* f = std::fopen(e->tune_cache_path.c_str(), "a")) {
+    std::fprintf(f, "%s tile=%u rows_per_item=%u hsum_rows=%u\n", key.c_str(), sh.tile, sh.rows_per_item,
+                 sh.hsum_rows);
+    std::fclose(f);
+  }
+}
+
+// Chooses (tile, rows per item, rows per summing workgroup) of a shape.
+// Configured values are kept; the rest comes from the tuning cache, else from
+// timing (may_time), else from the cost model.  Caller holds api_mu.
+fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32_t part, bool may_time,
+                       Shape* out) {
   const Bank& bank = *e->banks[bank_id];
   const uint32_t T = bank.n_templates;
   uint32_t rb, nr;
@@ -690,78 +915,45 @@ fdas_status tune_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank_id, uint32
     for (uint32_t s : kTileSizes)
       if (tile_usable(s, bank.max_extent)) tiles.push_back(s);
   }
-  std::vector<uint32_t> rpis = e->cfg.rows_per_item ? std::vector<uint32_t>{e->cfg.rows_per_item}
-                                                    : std::vector<uint32_t>{8, 16, 32, 64};
-  // Rows per summing workgroup are not timed: the timing runs on a zero spectrum,
-  // which has no detections, while on real data shorter chunks bound the tail of
-  // the workgroups that meet dense detections (11 rows measured best there).
-  // Defaults used when nothing needs timing.
-  out->tile = tiles.size() == 1 ? tiles[0] : (std::find(tiles.begin(), tiles.end(), 2048u) != tiles.end() ? 2048u : tiles[0]);
-  out->rows_per_item = rpis.size() == 1 ? rpis[0] : 16;
-  out->hsum_rows = e->cfg.hsum_rows ? e->cfg.hsum_rows : kHsumRowsPerBlock;
-  if (tiles.size() > 1 || rpis.size() > 1) {
-    const uint32_t spill = e->cfg.spill_per_harmonic ? e->cfg.spill_per_harmonic : default_spill(T, n_bins);
-    const TrialLayout lay = trial_layout(n_bins, T, 8, e->cfg.cands_per_harmonic, spill, false);
-    hipError_t herr = hipSuccess;
-    char* block = e->arena->alloc(lay.total, &herr);
-    char* mem = block ? block + lay.small_bytes : nullptr;
-    if (!block) {
-      info(e, "autotune of N=%u bank %u skipped: no room in the HBM budget; using tile %u", n_bins, bank_id,
-           out->tile);
-    } else {
-      GroupBuf g;
-      fdas_status st = group_reserve(e, g, kMaxRowGroups, 1, 0);
-      float best = FLT_MAX;
-      for (uint32_t tile : tiles) {
-        for (uint32_t rpi : rpis) {
-          if (st != FDAS_OK) break;
-          uint32_t n_items = 0;
-          std::vector<FtcJob> jobs = shape_jobs(bank, tile, n_bins, rb, nr, rpi, &n_items);
-          for (FtcJob& j : jobs) {
-            j.spectrum = e->zero_input;
-            j.fop = reinterpret_cast<float*>(mem + lay.fop);
-            j.hperm = bank.spectra.at(tile);
-          }
-          if (hipMemcpy(g.d_ftc, jobs.data(), sizeof(FtcJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess) {
-            st = fail(e, FDAS_EDEVICE, "job upload failed while autotuning");
-            break;
-          }
-          const auto& tw = e->twiddles.at(tile);
-          float ms = FLT_MAX;
-          st = time_best(
-              e,
-              [&]() -> fdas_status {
-                EHIP(e, launch_ftconv(int(tile), g.d_ftc, uint32_t(jobs.size()), n_items, tw.first, tw.second,
-                                      ftc_grid(e, n_items), e->s_ftc));
-                return FDAS_OK;
-              },
-              2, &ms);
-          info(e, "autotune N=%u bank %u: tile %u, %u rows/item: %.4f ms", n_bins, bank_id, tile, rpi, ms);
-          if (st == FDAS_OK && ms < best) {
-            best = ms;
-            out->tile = tile;
-            out->rows_per_item = rpi;
-          }
-        }
-      }
-      free_group(g);
-      e->arena->free(block);
-      if (st != FDAS_OK) return st;
-    }
+  const std::vector<uint32_t> rpis = e->cfg.rows_per_item ? std::vector<uint32_t>{e->cfg.rows_per_item}
+                                                          : std::vector<uint32_t>{8, 16, 32, 64};
+  const std::vector<uint32_t> hrows = e->cfg.hsum_rows ? std::vector<uint32_t>{e->cfg.hsum_rows}
+                                                       : std::vector<uint32_t>{11, 16, 22};
+  model_shape(e, bank, n_bins, rb, nr, tiles, out);
+  const std::string key = tune_key(e, n_bins, bank, part);
+  const char* how = "cost model";
+  auto cached = e->tune_cache.find(key);
+  if (cached != e->tune_cache.end() &&
+      std::find(tiles.begin(), tiles.end(), std::get<0>(cached->second)) != tiles.end()) {
+    out->tile = std::get<0>(cached->second);
+    out->rows_per_item = e->cfg.rows_per_item ? e->cfg.rows_per_item : std::get<1>(cached->second);
+    out->hsum_rows = e->cfg.hsum_rows ? e->cfg.hsum_rows : std::get<2>(cached->second);
+    how = "tuning cache";
+  } else if (tiles.size() == 1 && rpis.size() == 1 && hrows.size() == 1) {
+    out->tile = tiles[0];
+    out->rows_per_item = rpis[0];
+    out->hsum_rows = hrows[0];
+    how = "configuration";
+  } else if (may_time) {
+    const fdas_status st = time_shape(e, n_bins, bank_id, part, tiles, rpis, hrows, out);
+    if (st != FDAS_OK) return st;
+    save_tuned(e, key, *out);
+    how = "timing";
   }
   out->ftc = shape_jobs(bank, out->tile, n_bins, rb, nr, out->rows_per_item, &out->n_items);
-  info(e, "shape N=%u bank %u part %u: tile %u, %u rows/item, %u rows/summing workgroup, %zu row groups", n_bins,
-       bank_id, part, out->tile, out->rows_per_item, out->hsum_rows, out->ftc.size());
+  info(e, "shape N=%u bank %u part %u (%s): tile %u, %u rows/item, %u rows/summing workgroup, %zu row groups",
+       n_bins, bank_id, part, how, out->tile, out->rows_per_item, out->hsum_rows, out->ftc.size());
   return FDAS_OK;
 }
 
-// Shape of a request, tuning it on first use (caller holds api_mu).
-fdas_status get_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part, const Shape** out) {
+// Shape of a request, fixed on first use (caller holds api_mu).
+fdas_status get_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part, bool may_time,
+                      const Shape** out) {
   const auto key = std::make_tuple(n_bins, bank, part);
   auto it = e->shapes.find(key);
   if (it == e->shapes.end()) {
     auto sh = std::make_unique<Shape>();
-    const fdas_status st = tune_shape(e, n_bins, bank, part, sh.get());
+    const fdas_status st = tune_shape(e, n_bins, bank, part, may_time, sh.get());
     if (st != FDAS_OK) return st;
     it = e->shapes.emplace(key, std::move(sh)).first;
   }
@@ -775,7 +967,7 @@ fdas_status make_trial(fdas_engine* e, const fdas_request& q, bool own_input, Tr
   fdas_status st = validate_request(e, &q);
   if (st != FDAS_OK) return st;
   const Shape* sh = nullptr;
-  if ((st = get_shape(e, q.n_bins, q.bank, q.part, &sh)) != FDAS_OK) return st;
+  if ((st = get_shape(e, q.n_bins, q.bank, q.part, /*may_time=*/false, &sh)) != FDAS_OK) return st;
   t->req = q;
   t->shape = sh;
   t->bank = e->banks[q.bank].get();
@@ -835,7 +1027,8 @@ void fail_trial(fdas_engine* e, Trial& t, fdas_status st) {
 
 // Admits queued trials into a free context and launches them.  Returns true if
 // anything was admitted or failed (progress).
-bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight) {
+// mem_wait is set when the oldest queued trial waits for arena memory.
+bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight, bool& mem_wait) {
   std::vector<Trial> picked;
   {
     std::lock_guard<std::mutex> lk(e->q_mu);
@@ -853,8 +1046,11 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight) {
     hipError_t herr = hipSuccess;
     char* mem = e->arena->alloc(t.lay.total, &herr);
     if (!mem) {
-      if (picked.empty() && !others_in_flight) {
-        // Nothing else holds memory: this trial can never fit.
+      // Impossible when the trial alone exceeds the budget, or when nothing
+      // else holds memory (no group in flight, no live plan) and it still
+      // does not fit.  Otherwise it waits for memory to come back.
+      const bool never = t.lay.total > e->arena->budget() || (picked.empty() && !others_in_flight && e->live_plans == 0);
+      if (never) {
         {
           std::lock_guard<std::mutex> lk(e->q_mu);
           e->queue.pop_front();
@@ -866,6 +1062,7 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight) {
         progress = true;
         continue;
       }
+      mem_wait = true;
       break;  // wait for memory to come back
     }
     {
@@ -948,12 +1145,12 @@ void scheduler_loop(fdas_engine* e) {
       complete_ctx(e, *oldest);
       progress = true;
     }
-    bool any_busy = false, any_free = false;
+    bool any_busy = false, any_free = false, mem_wait = false;
     for (auto& c : e->ctxs) (c.busy ? any_busy : any_free) = true;
     if (any_free) {
       for (auto& c : e->ctxs) {
         if (c.busy) continue;
-        if (admit(e, c, any_busy)) {
+        if (admit(e, c, any_busy, mem_wait)) {
           progress = true;
           any_busy = true;
         }
@@ -963,8 +1160,11 @@ void scheduler_loop(fdas_engine* e) {
     if (progress) continue;
     std::unique_lock<std::mutex> lk(e->q_mu);
     if (e->stop && e->queue.empty() && !any_busy) break;
-    if (!any_busy) {
+    if (!any_busy && !mem_wait) {
       e->q_cv.wait(lk, [&] { return e->stop || !e->queue.empty(); });
+    } else if (!any_busy) {
+      // Memory is held by live plans: look again when one is destroyed (or after 1 ms).
+      e->q_cv.wait_for(lk, std::chrono::milliseconds(1));
     } else {
       e->q_cv.wait_for(lk, std::chrono::microseconds(50));
     }
@@ -998,6 +1198,11 @@ static fdas_status engine_init(fdas_engine* e) {
   int n_cu = 0;
   EHIP(e, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
   e->n_cu = uint32_t(n_cu);
+  hipDeviceProp_t prop{};
+  EHIP(e, hipGetDeviceProperties(&prop, c.device));
+  e->device_key = std::string(prop.name) + "/" + prop.gcnArchName + "/" + std::to_string(n_cu) + "CU";
+  for (char& ch : e->device_key)
+    if (ch == ' ' || ch == '\t' || ch == '|') ch = '_';
   EHIP(e, hipStreamCreateWithFlags(&e->s_ftc, hipStreamNonBlocking));
   EHIP(e, hipStreamCreateWithFlags(&e->s_hsum, hipStreamNonBlocking));
   for (uint32_t tile : kTileSizes) {
@@ -1030,6 +1235,7 @@ fdas_status fdas_engine_create(const fdas_engine_config* cfg_in, fdas_log_fn log
     return fail(e.get(), FDAS_EINVAL, "tile %u is not 1024, 2048 or 4096", c.tile);
   if (c.rows_per_item > 256) return fail(e.get(), FDAS_EINVAL, "rows_per_item %u above 256", c.rows_per_item);
   if (c.hsum_rows > 256) return fail(e.get(), FDAS_EINVAL, "hsum_rows %u above 256", c.hsum_rows);
+  if (c.no_fusion > 1) return fail(e.get(), FDAS_EINVAL, "no_fusion %u is not 0 or 1", c.no_fusion);
   if (c.ftc_workgroups_per_cu > 64)
     return fail(e.get(), FDAS_EINVAL, "ftc_workgroups_per_cu %u above 64", c.ftc_workgroups_per_cu);
   if (c.cands_per_harmonic < 1 || c.cands_per_harmonic > kSelectMaxCap)
@@ -1074,7 +1280,7 @@ void fdas_engine_destroy(fdas_engine* e) {
     (void)hipFree(kv.second.first);
     (void)hipFree(kv.second.second);
   }
-  (void)hipFree(e->zero_input);
+  (void)hipFree(e->tune_input);
   e->arena.reset();
   if (e->s_ftc) (void)hipStreamDestroy(e->s_ftc);
   if (e->s_hsum) (void)hipStreamDestroy(e->s_hsum);
@@ -1129,26 +1335,68 @@ fdas_status fdas_engine_add_bank(fdas_engine* e, const float* taps_c64, uint32_t
   return FDAS_OK;
 }
 
-fdas_status fdas_engine_prepare(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part, uint32_t* tile) {
-  if (!e) return FDAS_EINVAL;
+static fdas_status prepare_shape(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part, bool may_time,
+                                 uint32_t* tile) {
   std::lock_guard<std::mutex> lk(e->api_mu);
   EHIP(e, hipSetDevice(e->cfg.device));
   if (n_bins < 1 || n_bins > kMaxBins) return fail(e, FDAS_EINVAL, "n_bins %u outside 1 .. %u", n_bins, kMaxBins);
   if (bank >= e->banks.size()) return fail(e, FDAS_EINVAL, "unknown bank %u", bank);
   if (part > FDAS_ALL) return fail(e, FDAS_EINVAL, "invalid part %u", part);
-  if (e->zero_input_bins < n_bins) {
-    (void)hipFree(e->zero_input);
-    e->zero_input = nullptr;
-    e->zero_input_bins = 0;
-    fdas_status st = dev_alloc(e, &e->zero_input, kMaxBins);
-    if (st != FDAS_OK) return st;
-    EHIP(e, hipMemset(e->zero_input, 0, sizeof(float2) * kMaxBins));
-    e->zero_input_bins = kMaxBins;
-  }
   const Shape* sh = nullptr;
-  const fdas_status st = get_shape(e, n_bins, bank, part, &sh);
+  const fdas_status st = get_shape(e, n_bins, bank, part, may_time, &sh);
   if (st == FDAS_OK && tile) *tile = sh->tile;
   return st;
+}
+
+fdas_status fdas_engine_prepare(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part, uint32_t* tile) {
+  if (!e) return FDAS_EINVAL;
+  return prepare_shape(e, n_bins, bank, part, /*may_time=*/true, tile);
+}
+
+fdas_status fdas_engine_shape_info(fdas_engine* e, uint32_t n_bins, uint32_t bank, uint32_t part,
+                                   fdas_shape_info* out) {
+  if (!e) return FDAS_EINVAL;
+  if (!out) return fail(e, FDAS_EINVAL, "null shape info");
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  auto it = e->shapes.find(std::make_tuple(n_bins, bank, part));
+  if (it == e->shapes.end())
+    return fail(e, FDAS_EINVAL, "shape N=%u bank %u part %u has not been prepared", n_bins, bank, part);
+  const Shape& sh = *it->second;
+  *out = fdas_shape_info{};
+  out->tile = sh.tile;
+  out->rows_per_item = sh.rows_per_item;
+  out->hsum_rows = sh.hsum_rows;
+  out->n_row_groups = uint32_t(sh.ftc.size());
+  for (const FtcJob& j : sh.ftc) {
+    out->forward_transforms += uint64_t(j.n_tiles) * j.n_chunks;
+    out->inverse_transforms += uint64_t(j.n_tiles) * j.n_rows;
+    out->stored_outputs += uint64_t(j.n_rows) * n_bins;
+  }
+  return FDAS_OK;
+}
+
+fdas_status fdas_engine_tune_cache(fdas_engine* e, const char* path) {
+  if (!e) return FDAS_EINVAL;
+  if (!path || !*path) return fail(e, FDAS_EINVAL, "empty tuning-cache path");
+  std::lock_guard<std::mutex> lk(e->api_mu);
+  e->tune_cache_path = path;
+  uint32_t n = 0;
+  if (FILE* f = std::fopen(path, "r")) {
+    char line[512];
+    while (std::fgets(line, sizeof(line), f)) {
+      char key[400];
+      unsigned tile = 0, rpi = 0, rows = 0;
+      if (std::sscanf(line, "%399s tile=%u rows_per_item=%u hsum_rows=%u", key, &tile, &rpi, &rows) != 4) continue;
+      const bool ok = (tile == 1024 || tile == 2048 || tile == 4096) && rpi >= 1 && rpi <= 256 && rows >= 1 &&
+                      rows <= 256;
+      if (!ok) continue;
+      e->tune_cache[key] = std::make_tuple(uint32_t(tile), uint32_t(rpi), uint32_t(rows));  // later lines win
+      ++n;
+    }
+    std::fclose(f);
+  }
+  info(e, "tuning cache %s: %u tuned shapes", path, n);
+  return FDAS_OK;
 }
 
 // ---------------------------------------------------------------- plans
@@ -1167,6 +1415,7 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
   auto p = std::make_unique<fdas_plan>();
   p->eng = e;
   p->use_graph = (flags & FDAS_PLAN_GRAPH) != 0;
+  p->fused = e->cfg.no_fusion == 0;
   p->trials.resize(n);
   for (uint32_t i = 0; i < n; ++i) {
     fdas_status st = make_trial(e, reqs[i], false, &p->trials[i]);
@@ -1254,6 +1503,7 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
     return cleanup(fail(e, FDAS_EDEVICE, "event creation failed"));
   info(e, "plan: %u trials in %u groups, %u groups resident (%.2f GB)", n, n_groups, n_regions,
        double(p->region_bytes) / 1e9);
+  e->live_plans += 1;
   *out = p.release();
   return FDAS_OK;
 }
@@ -1286,8 +1536,76 @@ fdas_status fdas_plan_set_request(fdas_plan* p, uint32_t i, const void* spectrum
   return FDAS_OK;
 }
 
+// Lagged schedule of a plan's G groups on the convolution stream A: launch g
+// (g = 0..G) runs group g's FT convolution together with group g-1's harmonic
+// summing, as one fused kernel when the pair's shapes allow it (1024-point
+// tiles, one summing shape), else as consecutive kernels.  Group g-1's
+// selection and read-back follow on stream B beside launch g+1.  Timing events
+// (eager runs only) bracket each launch: ev_ftc0 / ev_ftc_t1 of group g for
+// launches 0..G-1, ev_hsum0 / ev_hsum1 of the last group for the final one.
+static fdas_status plan_enqueue_lagged(fdas_plan* p, bool capturing) {
+  fdas_engine* e = p->eng;
+  hipStream_t A = e->s_ftc, B = e->s_hsum;
+  const uint32_t R = uint32_t(p->regions.size()), G = uint32_t(p->groups.size());
+  std::vector<std::vector<Trial*>> ptrs(G);
+  for (uint32_t g = 0; g < G; ++g)
+    for (uint32_t i : p->groups[g]) ptrs[g].push_back(&p->trials[i]);
+  for (uint32_t g = 0; g <= G; ++g) {
+    GroupBuf* cur = g < G ? &p->bufs[g] : nullptr;    // convolved in this launch
+    GroupBuf* prev = g > 0 ? &p->bufs[g - 1] : nullptr;  // summed in this launch
+    if (cur && g >= R) EHIP(e, hipStreamWaitEvent(A, p->bufs[g - R].ev_done, 0));  // its region is free again
+    if (prev) {
+      if (prev->small_span) {
+        EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(prev->small_span), uint32_t(prev->span_bytes / 4), A));
+      } else {
+        for (const Trial* t : ptrs[g - 1])
+          EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(t->small), uint32_t(t->lay.small_bytes / 4), A));
+      }
+    }
+    hipEvent_t t0 = cur ? cur->ev_ftc0 : prev->ev_hsum0, t1 = cur ? cur->ev_ftc_t1 : prev->ev_hsum1;
+    if (!capturing) EHIP(e, hipEventRecord(t0, A));
+    const bool fuse = cur && prev && cur->ftc.size() == 1 && prev->hsum.size() == 1 &&
+                      cur->ftc[0].tile == 1024 &&
+                      fused_supported(1024, prev->hsum[0].H, prev->hsum[0].vec);
+    if (cur) cur->launch_fused = fuse;
+    if (fuse) {
+      const auto& F = cur->ftc[0];
+      const auto& S = prev->hsum[0];
+      const auto& tw = e->twiddles.at(1024);
+      EHIP(e, launch_fused(cur->d_ftc + F.first, F.n_jobs, F.n_items, tw.first, tw.second, prev->d_hsum + S.first,
+                           S.n_jobs, S.n_blocks, S.H, S.vec, A));
+    } else {
+      if (cur)
+        for (const auto& L : cur->ftc) {
+          const auto& tw = e->twiddles.at(L.tile);
+          EHIP(e, launch_ftconv(int(L.tile), cur->d_ftc + L.first, L.n_jobs, L.n_items, tw.first, tw.second,
+                                ftc_grid(e, L.n_items), A));
+        }
+      if (prev)
+        for (const auto& L : prev->hsum)
+          EHIP(e, launch_hsum(prev->d_hsum + L.first, L.n_jobs, L.n_blocks, L.H, L.vec, A));
+    }
+    if (!capturing) EHIP(e, hipEventRecord(t1, A));
+    if (prev) {
+      EHIP(e, hipEventRecord(prev->ev_hsum_done, A));
+      EHIP(e, hipStreamWaitEvent(B, prev->ev_hsum_done, 0));
+      EHIP(e, launch_select(prev->d_sel, prev->n_sel, B));
+      if (prev->small_span) {
+        EHIP(e, hipMemcpyAsync(prev->h_span, prev->small_span, prev->span_bytes, hipMemcpyDeviceToHost, B));
+      } else {
+        for (const Trial* t : ptrs[g - 1])
+          EHIP(e, hipMemcpyAsync(t->h_read, t->small, t->lay.readback_bytes, hipMemcpyDeviceToHost, B));
+      }
+      EHIP(e, hipEventRecord(prev->ev_done, B));
+    }
+  }
+  EHIP(e, hipStreamWaitEvent(A, p->bufs.back().ev_done, 0));
+  return FDAS_OK;
+}
+
 static fdas_status plan_enqueue(fdas_plan* p, bool capturing) {
   fdas_engine* e = p->eng;
+  if (p->fused) return plan_enqueue_lagged(p, capturing);
   const uint32_t R = uint32_t(p->regions.size());
   for (uint32_t g = 0; g < p->groups.size(); ++g) {
     std::vector<Trial*> ptrs;
@@ -1374,11 +1692,22 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p->ev_start, p->ev_end) == hipSuccess) out->last_ms = ms;
     (void)hipGetLastError();
-    for (const GroupBuf& g : p->bufs) {
+    out->lagged = p->fused ? 1u : 0u;
+    for (size_t gi = 0; gi < p->bufs.size(); ++gi) {
       if (!p->eager_timed) break;
+      const GroupBuf& g = p->bufs[gi];
       float a = 0.f, b = 0.f;
-      if (hipEventElapsedTime(&a, g.ev_ftc0, g.ev_ftc_t1) == hipSuccess) out->ftc_ms += a;
-      if (hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
+      if (hipEventElapsedTime(&a, g.ev_ftc0, g.ev_ftc_t1) == hipSuccess) {
+        if (p->fused && g.launch_fused) {
+          out->fused_ms += a;
+          out->n_fused += 1;
+        } else {
+          out->ftc_ms += a;
+        }
+      }
+      (void)hipGetLastError();
+      const bool summed_here = !p->fused || gi + 1 == p->bufs.size();  // lagged: the last group's own launch
+      if (summed_here && hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
       (void)hipGetLastError();
     }
   }
@@ -1400,7 +1729,9 @@ void fdas_plan_destroy(fdas_plan* p) {
   (void)hipHostFree(p->h_read);
   if (p->ev_start) (void)hipEventDestroy(p->ev_start);
   if (p->ev_end) (void)hipEventDestroy(p->ev_end);
+  e->live_plans -= 1;
   delete p;
+  e->q_cv.notify_all();  // a served trial may be waiting for the memory just returned
 }
 
 // ---------------------------------------------------------------- serving
@@ -1411,7 +1742,15 @@ fdas_status fdas_engine_submit(fdas_engine* e, const fdas_request* req, fdas_res
   if (!req) return fail(e, FDAS_EINVAL, "null request");
   Trial t;
   {
-    fdas_status st = fdas_engine_prepare(e, req->n_bins, req->bank, req->part, nullptr);
+    // A shape first seen while requests are being served is not timed (that
+    // would stall the scheduler and skew its timings): it takes the tuning
+    // cache's choice or the cost model's.  fdas_engine_prepare times it beforehand.
+    bool serving;
+    {
+      std::lock_guard<std::mutex> lk(e->q_mu);
+      serving = e->running;
+    }
+    fdas_status st = prepare_shape(e, req->n_bins, req->bank, req->part, !serving, nullptr);
     if (st != FDAS_OK) return st;
     std::lock_guard<std::mutex> lk(e->api_mu);
     if ((st = make_trial(e, *req, true, &t)) != FDAS_OK) return st;

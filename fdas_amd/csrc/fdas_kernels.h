@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "fdas_jobs.h"
+#include "fdas_select.h"
 
 namespace fdas {
 
@@ -19,7 +20,6 @@ constexpr int kHsumWaveStage = 64;        // LDS staging entries per plane per w
 constexpr int kSelectThreads = 1024;
 constexpr uint32_t kSelectMaxCap = 8192;  // largest per-plane cap the in-LDS sort handles
 constexpr uint32_t kSelectPool = 4096;    // boundary-bin entries resolved exactly in LDS
-constexpr uint32_t kSelectBins = 2048;    // power histogram bins (256 per octave above the threshold)
 constexpr uint32_t kSelectSlices = 64;    // workgroups per plane in the histogram / compaction passes
 
 int ftconv_threads(int tile);
@@ -53,5 +53,14 @@ hipError_t launch_hmax(const float* fop, uint32_t n_bins, uint32_t n_templates, 
                        uint32_t n_harmonics, float* out, hipStream_t stream);
 
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
+
+// One launch running both the FT convolution of ftc_jobs (1024-point tiles)
+// and the harmonic summing of hsum_jobs (n_harmonics planes; vec as in
+// launch_hsum) over a FOP written by an earlier launch; workgroups of the two
+// kinds are interleaved.  fused_supported tells which shapes it covers.
+bool fused_supported(uint32_t tile, uint32_t n_harmonics, bool vec);
+hipError_t launch_fused(const FtcJob* ftc_jobs, uint32_t n_ftc_jobs, uint32_t n_items, const float2* tw1,
+                        const float2* tw2, const HsumJob* hsum_jobs, uint32_t n_hsum_jobs, uint32_t n_blocks,
+                        uint32_t n_harmonics, bool vec, hipStream_t stream);
 
 }  // namespace fdas

@@ -111,6 +111,104 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
   }
 }
 
+// 1024-point tiles, four waves per workgroup sharing one tile.  Wave 0 loads
+// the tile and forward-transforms it into LDS (xs); then every wave convolves
+// its share of the item's template rows (rows w, w + 4, ...), each row reading
+// the tile spectrum back from LDS.  Keeping the spectrum in LDS instead of
+// registers, and exchanging the F2 <-> F3 lane bits with permlane swaps
+// instead of LDS, brings the kernel to three waves per SIMD.
+constexpr int kFtc1kWaves = 4;
+
+// LDS of one 1024-point work item, carved from a caller's buffer.
+struct Ftc1kLds {
+  float2 xs[16 * WaveFFT1k::NT];                   // the tile's spectrum, slot-major
+  float2 lbuf[kFtc1kWaves][WaveFFT1k::LDS_ELEMS];  // each wave's exchange buffer
+  float2 w2s[64];                                  // W_64^(d * kc), the F2 twiddles
+};
+
+// One work item (tile, chunk of template rows) of a 1024-point job list.
+// `fft` holds this thread's twiddles; `first` is true for the workgroup's
+// first item (wave 0 has just filled w2s: the first barrier publishes it).
+__device__ __forceinline__ void ftc1k_item(Ftc1kLds& L, const WaveFFT1k& fft, const FtcJob* __restrict__ jobs,
+                                           uint32_t n_jobs, uint32_t item) {
+  using F = WaveFFT1k;
+  constexpr int NT = F::NT;
+  constexpr int S = F::S;
+  const int lane = int(threadIdx.x) & 63;
+  const uint32_t wave = threadIdx.x >> 6;
+  float2* __restrict__ lb = L.lbuf[wave];
+  const FtcJob* job = jobs + find_job_ftc(jobs, n_jobs, item);
+  const uint32_t n_bins = job->n_bins;
+  const uint32_t overlap = job->overlap;
+  const uint32_t payload = job->payload;
+  const uint32_t local = item - job->item_begin;
+  const uint32_t chunk = local / job->n_tiles;  // consecutive items share a template chunk
+  const uint32_t tile = local - chunk * job->n_tiles;
+  const uint32_t r_begin = job->row_begin + chunk * job->rows_per_item;
+  uint32_t r_end = r_begin + job->rows_per_item;
+  const uint32_t r_stop = job->row_begin + job->n_rows;
+  if (r_end > r_stop) r_end = r_stop;
+  const float2* __restrict__ hbase = static_cast<const float2*>(job->hperm);
+  const uint32_t out0 = tile * payload;  // FOP bin of tile sample s = overlap
+
+  // This wave's first template row is loaded before the forward transform,
+  // every next one as soon as the current one has been used.
+  uint32_t r = r_begin + wave;
+  float2 hn[16];
+  {
+    const float2* __restrict__ h = hbase + size_t(min(r, r_end - 1)) * S + lane;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) hn[s] = gld2(h, size_t(s * NT));
+  }
+
+  __syncthreads();  // every wave has finished reading the previous item's spectrum
+  if (wave == 0) {
+    const float2* __restrict__ x = static_cast<const float2*>(job->spectrum);
+    float2 v[16];
+    const int base = int(tile * payload) - int(overlap);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int idx = base + NT * a + lane;
+      v[a] = make_float2(0.f, 0.f);
+      if (idx >= 0 && idx < int(n_bins)) v[a] = gld2(x, size_t(idx));
+    }
+    fft.forward(v, lb, lane);
+#pragma unroll
+    for (int s = 0; s < 16; ++s) L.xs[s * NT + lane] = v[s];
+  }
+  __syncthreads();
+
+  for (; r < r_end; r += kFtc1kWaves) {
+    float2 v[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) v[s] = cmul(L.xs[s * NT + lane], hn[s]);
+    {
+      const float2* __restrict__ h = hbase + size_t(min(r + kFtc1kWaves, r_end - 1)) * S + lane;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) hn[s] = gld2(h, size_t(s * NT));
+    }
+    fft.inverse(v, lb, lane);
+    float* __restrict__ out = job->fop + size_t(r) * n_bins + (ptrdiff_t(out0) + lane - ptrdiff_t(overlap));
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const uint32_t s = uint32_t(NT * a + lane);
+      if (s >= overlap) {
+        const uint32_t f = out0 + s - overlap;
+        if (f < n_bins) gst_stream(out, size_t(NT * a), v[a].x * v[a].x + v[a].y * v[a].y);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64 * kFtc1kWaves) __attribute__((amdgpu_waves_per_eu(3, 3))) void ftconv1k_kernel(
+    const FtcJob* __restrict__ jobs, uint32_t n_jobs, uint32_t n_items, const float2* __restrict__ tw1,
+    const float2* __restrict__ tw2) {
+  __shared__ Ftc1kLds L;
+  WaveFFT1k fft;
+  fft.load_twiddles(tw1, tw2, L.w2s, int(threadIdx.x) & 63, (threadIdx.x >> 6) == 0);
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) ftc1k_item(L, fft, jobs, n_jobs, item);
+}
+
 template <int S>
 __global__ __launch_bounds__(S / 16) void tiles_kernel(const float2* __restrict__ x, uint32_t n_bins,
                                                        uint32_t overlap, uint32_t payload,
@@ -136,14 +234,14 @@ __global__ __launch_bounds__(S / 16) void tiles_kernel(const float2* __restrict_
   for (int s = 0; s < 16; ++s) o[F::slot_bin(tid, s)] = v[s];
 }
 
-int ftconv_threads(int tile) { return tile / 16; }
+int ftconv_threads(int tile) { return tile == 1024 ? 64 * kFtc1kWaves : tile / 16; }
 
 hipError_t launch_ftconv(int tile, const FtcJob* d_jobs, uint32_t n_jobs, uint32_t n_items, const float2* tw1,
                          const float2* tw2, uint32_t grid, hipStream_t stream) {
   if (n_items == 0) return hipSuccess;
   if (grid == 0 || grid > n_items) grid = n_items;
   switch (tile) {
-    case 1024: hipLaunchKernelGGL(ftconv_kernel<1024>, dim3(grid), dim3(64), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
+    case 1024: hipLaunchKernelGGL(ftconv1k_kernel, dim3(grid), dim3(64 * kFtc1kWaves), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
     case 2048: hipLaunchKernelGGL(ftconv_kernel<2048>, dim3(grid), dim3(128), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
     case 4096: hipLaunchKernelGGL(ftconv_kernel<4096>, dim3(grid), dim3(256), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
     default: return hipErrorInvalidValue;
@@ -365,22 +463,22 @@ __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, c
   }
 }
 
-// HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] in FP32, in k order (hsum.py:331);
+// HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] in FP32, in k order (hsum.py:93-120);
 // strict HP_k > thr_k (detect.cl.mako:81-83).  One workgroup covers 2048 bins
 // of a chunk of rows of its part, each thread 8 consecutive bins.  Rows are
 // processed in pairs with two register buffers: the loads of row r+1 are in
 // flight while row r is summed and thresholded.
 template <int H, bool VEC>
-__global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __restrict__ jobs, uint32_t n_jobs) {
-  __shared__ HsumStage<H> st;
-  const HsumJob& job = jobs[find_job_hsum(jobs, n_jobs, blockIdx.x)];
+__device__ __forceinline__ void hsum_block(HsumStage<H>& st, const HsumJob* __restrict__ jobs, uint32_t n_jobs,
+                                           uint32_t blk) {
+  const HsumJob& job = jobs[find_job_hsum(jobs, n_jobs, blk)];
   const int tid = threadIdx.x;
   const uint32_t wave = uint32_t(tid) >> 6;
 
   const uint32_t n_bins = job.n_bins;
   const int c = int(job.n_templates / 2);
   const float* __restrict__ fop = job.fop;
-  const uint32_t local = blockIdx.x - job.block_begin;
+  const uint32_t local = blk - job.block_begin;
   const uint32_t chunk = local % job.n_row_chunks;  // row chunks of one bin block run side by side
   const uint32_t f0 = (local / job.n_row_chunks) * uint32_t(kHsumBinsPerBlock) + uint32_t(tid * kHsumBinsPerThread);
   // VEC: threads past the end read (and discard) the last 8 bins; both values are multiples of 8
@@ -434,6 +532,12 @@ __global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __res
   }
 }
 
+template <int H, bool VEC>
+__global__ __launch_bounds__(kHsumThreads) void hsum_kernel(const HsumJob* __restrict__ jobs, uint32_t n_jobs) {
+  __shared__ HsumStage<H> st;
+  hsum_block<H, VEC>(st, jobs, n_jobs, blockIdx.x);
+}
+
 template <int H>
 static void launch_hsum_h(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks, bool vec, hipStream_t stream) {
   if (vec)
@@ -464,6 +568,65 @@ hipError_t launch_hsum(const HsumJob* d_jobs, uint32_t n_jobs, uint32_t n_blocks
     case 16: launch_hsum_h<16>(d_jobs, n_jobs, n_blocks, vec, stream); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fused launch: FT convolution of one group with 1024-point tiles and the
+// harmonic summing of the group before it
+// ---------------------------------------------------------------------------
+
+// Workgroup v is either an FT-convolution work item or a summing block; the
+// two kinds are interleaved evenly over the grid (v is an FT item iff
+// floor((v+1) A / (A+B)) > floor(v A / (A+B)) for A items and B blocks), so
+// the hardware's in-order dispatch keeps both in flight together: the
+// VALU-bound transforms and the HBM-bound summing share every CU instead of
+// taking turns.  The summing reads a FOP that an earlier launch wrote, so no
+// workgroup waits on another.  LDS is one buffer viewed as either kind's.
+template <int H>
+union FusedLds {
+  Ftc1kLds ftc;
+  HsumStage<H> hsum;
+};
+
+template <int H, bool VEC>
+__global__ __launch_bounds__(64 * kFtc1kWaves) __attribute__((amdgpu_waves_per_eu(3, 3))) void fused1k_kernel(
+    const FtcJob* __restrict__ ftc_jobs, uint32_t n_ftc_jobs, uint32_t n_items, const float2* __restrict__ tw1,
+    const float2* __restrict__ tw2, const HsumJob* __restrict__ hsum_jobs, uint32_t n_hsum_jobs,
+    uint32_t n_blocks) {
+  static_assert(kHsumThreads == 64 * kFtc1kWaves, "both kinds of work use the same workgroup size");
+  __shared__ FusedLds<H> lds;
+  const uint32_t v = blockIdx.x;
+  const uint64_t total = uint64_t(n_items) + n_blocks;
+  const uint32_t a0 = uint32_t(uint64_t(v) * n_items / total);
+  const uint32_t a1 = uint32_t(uint64_t(v + 1) * n_items / total);
+  if (a1 > a0) {
+    WaveFFT1k fft;
+    fft.load_twiddles(tw1, tw2, lds.ftc.w2s, int(threadIdx.x) & 63, (threadIdx.x >> 6) == 0);
+    ftc1k_item(lds.ftc, fft, ftc_jobs, n_ftc_jobs, a0);
+  } else {
+    hsum_block<H, VEC>(lds.hsum, hsum_jobs, n_hsum_jobs, v - a0);
+  }
+}
+
+bool fused_supported(uint32_t tile, uint32_t n_harmonics, bool vec) {
+  return tile == 1024 && (n_harmonics == 8 || (n_harmonics == 16 && vec));
+}
+
+hipError_t launch_fused(const FtcJob* ftc_jobs, uint32_t n_ftc_jobs, uint32_t n_items, const float2* tw1,
+                        const float2* tw2, const HsumJob* hsum_jobs, uint32_t n_hsum_jobs, uint32_t n_blocks,
+                        uint32_t n_harmonics, bool vec, hipStream_t stream) {
+  const uint32_t grid = n_items + n_blocks;
+  if (grid == 0) return hipSuccess;
+  const dim3 block(64 * kFtc1kWaves);
+#define FDAS_FUSED(HH, VV)                                                                                          \
+  hipLaunchKernelGGL((fused1k_kernel<HH, VV>), dim3(grid), block, 0, stream, ftc_jobs, n_ftc_jobs, n_items, tw1, tw2, \
+                     hsum_jobs, n_hsum_jobs, n_blocks)
+  if (n_harmonics == 8 && vec) FDAS_FUSED(8, true);
+  else if (n_harmonics == 8) FDAS_FUSED(8, false);
+  else if (n_harmonics == 16 && vec) FDAS_FUSED(16, true);
+  else return hipErrorInvalidValue;
+#undef FDAS_FUSED
   return hipGetLastError();
 }
 
@@ -583,19 +746,19 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_waves, uint32_t
   return res;
 }
 
-// Histogram bin of a detection's power: 256 bins per octave above the plane's
-// threshold (bits of non-negative floats are monotonic in the value).
-__device__ __forceinline__ uint32_t threshold_bits(float thr) { return thr > 0.f ? __float_as_uint(thr) : 0u; }
-__device__ __forceinline__ uint32_t power_bin(uint32_t pbits, uint32_t tbits) {
-  return pbits <= tbits ? 0u : min((pbits - tbits) >> 15, kSelectBins - 1);
-}
-
 // Cut bin b*: the highest bin whose suffix count sum_{b >= b*} hist[b] reaches
-// `cap`.  s_out[0] = b*, s_out[1] = entries strictly above b*.
+// `cap`.  s_out[0] = b*, s_out[1] = entries strictly above b*.  The histogram
+// counts every spilled entry and there are more than `cap` of them, so a cut
+// exists; the defaults (bin 0, nothing above) keep b* inside the histogram
+// regardless.
 template <int THREADS>
 __device__ void find_cut(const uint32_t* __restrict__ hist, uint32_t cap, uint32_t* s_waves, uint32_t* s_out) {
   constexpr int PER = int(kSelectBins) / THREADS;
   const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_out[0] = 0;
+    s_out[1] = 0;
+  }
   uint32_t h[PER];
   uint32_t local = 0;
 #pragma unroll
@@ -605,16 +768,10 @@ __device__ void find_cut(const uint32_t* __restrict__ hist, uint32_t cap, uint32
   }
   uint32_t total;
   const uint32_t excl = block_exclusive_scan<THREADS>(local, s_waves, &total);
-  if (excl < cap && cap <= excl + local) {
-    uint32_t acc = excl;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      if (acc < cap && cap <= acc + h[j]) {
-        s_out[0] = kSelectBins - 1 - uint32_t(tid * PER + j);
-        s_out[1] = acc;
-      }
-      acc += h[j];
-    }
+  uint32_t bin, above;
+  if (cut_in_share<PER>(h, excl, cap, uint32_t(tid * PER), &bin, &above)) {
+    s_out[0] = bin;
+    s_out[1] = above;
   }
   __syncthreads();
 }
@@ -627,20 +784,20 @@ __global__ __launch_bounds__(256) void cand_hist_kernel(const SelectJob* __restr
   const int tid = threadIdx.x;
   for (uint32_t i = tid; i < kSelectBins; i += 256) h[i] = 0;
   __syncthreads();
-  const uint32_t tb = threshold_bits(job.thr);
+  const uint32_t tb = histogram_base(__float_as_uint(job.thr));
   const uint32_t per = (n + kSelectSlices - 1) / kSelectSlices;
   const uint32_t beg = blockIdx.x * per, end = min(beg + per, n);
   const float* __restrict__ pw = job.spill_pwr;
   for (uint32_t e0 = beg; e0 < end; e0 += 4 * 256) {
-    float v[4];
+    uint32_t v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t e = e0 + uint32_t(u * 256 + tid);
-      v[u] = e < end ? gld(pw, e) : -1.f;
+      v[u] = e < end ? __float_as_uint(gld(pw, e)) : 0xffffffffu;  // all ones: past the slice
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (v[u] >= 0.f) atomicAdd(&h[power_bin(__float_as_uint(v[u]), tb)], 1u);
+      if (v[u] != 0xffffffffu) atomicAdd(&h[power_bin(v[u], tb)], 1u);
   }
   __syncthreads();
   for (uint32_t i = tid; i < kSelectBins; i += 256)
@@ -655,7 +812,7 @@ __global__ __launch_bounds__(256) void cand_compact_kernel(const SelectJob* __re
   __shared__ uint32_t cut[2];
   find_cut<256>(job.hist, job.cap, waves, cut);
   const uint32_t bstar = cut[0];
-  const uint32_t tb = threshold_bits(job.thr);
+  const uint32_t tb = histogram_base(__float_as_uint(job.thr));
   const int tid = threadIdx.x;
   const uint32_t per = (n + kSelectSlices - 1) / kSelectSlices;
   const uint32_t beg = blockIdx.x * per, end = min(beg + per, n);
@@ -795,17 +952,17 @@ __device__ uint32_t radix_select_spill(SelectShared& sh, const SelectJob& job, u
   const uint32_t cap = job.cap;
   const uint32_t* __restrict__ skey = job.spill_key;
   const float* __restrict__ spwr = job.spill_pwr;
-  auto pbits = [&](uint32_t e) { return __float_as_uint(gld(spwr, e)); };
+  auto pkey = [&](uint32_t e) { return order_key(__float_as_uint(gld(spwr, e))); };
   uint32_t prefix = 0, mask = 0, rank = cap;
-  radix_pass<true>(sh, n, pbits, 21, 11, prefix, mask, rank);
-  radix_pass<true>(sh, n, pbits, 10, 11, prefix, mask, rank);
-  radix_pass<true>(sh, n, pbits, 0, 10, prefix, mask, rank);
-  const uint32_t vbits = prefix;
+  radix_pass<true>(sh, n, pkey, 21, 11, prefix, mask, rank);
+  radix_pass<true>(sh, n, pkey, 10, 11, prefix, mask, rank);
+  radix_pass<true>(sh, n, pkey, 0, 10, prefix, mask, rank);
+  const uint32_t vkey = prefix;  // order key of the cap-th strongest power V
   const uint32_t n_equal = sh.misc[2];
   // `rank` entries with power == V are kept: the smallest keys among them.
   uint32_t kmax = 0xffffffffu;
   if (n_equal > rank) {
-    auto kbits = [&](uint32_t e) { return __float_as_uint(gld(spwr, e)) == vbits ? gld(skey, e) : 0xffffffffu; };
+    auto kbits = [&](uint32_t e) { return pkey(e) == vkey ? gld(skey, e) : 0xffffffffu; };
     uint32_t kp = 0, km = 0, kr = rank;
     radix_pass<false>(sh, n, kbits, 21, 11, kp, km, kr);
     radix_pass<false>(sh, n, kbits, 10, 11, kp, km, kr);
@@ -816,9 +973,9 @@ __device__ uint32_t radix_select_spill(SelectShared& sh, const SelectJob& job, u
   __syncthreads();
   for (uint32_t e = tid; e < n; e += kSelectThreads) {
     const float pv = gld(spwr, e);
-    const uint32_t pb = __float_as_uint(pv);
+    const uint32_t pk = order_key(__float_as_uint(pv));
     const uint32_t key = gld(skey, e);
-    if (pb > vbits || (pb == vbits && key <= kmax)) {
+    if (pk > vkey || (pk == vkey && key <= kmax)) {
       const uint32_t pos = atomicAdd(&sh.misc[3], 1u);
       if (pos < cap) {
         sh.key[pos] = key;
@@ -851,7 +1008,7 @@ __global__ __launch_bounds__(kSelectThreads) void select_final_kernel(const Sele
     const uint32_t pool_n = gld(job.hist, bstar);
     __syncthreads();
     if (m <= job.compact_cap && pool_n <= kSelectPool && above < cap) {
-      const uint32_t tb = threshold_bits(job.thr);
+      const uint32_t tb = histogram_base(__float_as_uint(job.thr));
       if (tid == 0) {
         sh.misc[3] = 0;
         sh.misc[4] = 0;
@@ -870,7 +1027,7 @@ __global__ __launch_bounds__(kSelectThreads) void select_final_kernel(const Sele
           }
         } else if (bin == bstar) {
           const uint32_t q = atomicAdd(&sh.misc[4], 1u);
-          if (q < kSelectPool) sh.pool[q] = (uint64_t(~pb) << 32) | key;
+          if (q < kSelectPool) sh.pool[q] = (uint64_t(~order_key(pb)) << 32) | key;  // strongest first
         }
       }
       __syncthreads();
@@ -880,7 +1037,7 @@ __global__ __launch_bounds__(kSelectThreads) void select_final_kernel(const Sele
       for (uint32_t i = tid; i < take && i < n_pool; i += kSelectThreads) {
         const uint64_t v = sh.pool[i];
         sh.key[above + i] = uint32_t(v);
-        sh.pwr[above + i] = __uint_as_float(~uint32_t(v >> 32));
+        sh.pwr[above + i] = __uint_as_float(order_key_inv(~uint32_t(v >> 32)));
       }
       __syncthreads();
       kept = cap;

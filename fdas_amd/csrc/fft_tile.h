@@ -341,4 +341,122 @@ struct TileFFT {
   }
 };
 
+
+// ---------------------------------------------------------------------------
+// One-wave 1024-point transform whose first exchange stays in registers.
+//
+// The same three passes as TileFFT<1024> (F1 DFT-16, F2 DFT-16, F3 DFT-4 and
+// the mirror-image inverse), with the same spectrum slot order, so template
+// spectra laid out for TileFFT<1024> serve both.  The exchange between F2 and
+// F3 only swaps the two high lane bits (lane / 16) with the two low register
+// bits (slot mod 4): thread (ka, m) slot 4i + dd <-> thread (ka, dd) slot
+// 4i + m, ka = lane mod 16.  gfx950's v_permlane16_swap (lane bit 4) and
+// v_permlane32_swap (lane bit 5) do each 2x2 block transpose in one
+// instruction per dword, without LDS.  The other exchange (F1 <-> F2) moves the
+// low lane bits and goes through the wave's own LDS buffer; it needs no
+// workgroup barrier, since only this wave reads what it wrote.
+// ---------------------------------------------------------------------------
+
+// 2x2 transpose of (lane bit 4, register pair): x holds the bit-0 register,
+// y the bit-1 register.  Odd 16-lane rows of x trade places with even rows of y.
+__device__ __forceinline__ void swap_lane_bit4(float2& x, float2& y) {
+  const auto re = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.x), __float_as_uint(y.x), false, false);
+  const auto im = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.y), __float_as_uint(y.y), false, false);
+  x = make_float2(__uint_as_float(re[0]), __uint_as_float(im[0]));
+  y = make_float2(__uint_as_float(re[1]), __uint_as_float(im[1]));
+}
+// The same for lane bit 5: the upper 32 lanes of x trade places with the lower 32 of y.
+__device__ __forceinline__ void swap_lane_bit5(float2& x, float2& y) {
+  const auto re = __builtin_amdgcn_permlane32_swap(__float_as_uint(x.x), __float_as_uint(y.x), false, false);
+  const auto im = __builtin_amdgcn_permlane32_swap(__float_as_uint(x.y), __float_as_uint(y.y), false, false);
+  x = make_float2(__uint_as_float(re[0]), __uint_as_float(im[0]));
+  y = make_float2(__uint_as_float(re[1]), __uint_as_float(im[1]));
+}
+
+// Thread (ka, m) slot 4i + dd <-> thread (ka, dd) slot 4i + m (an involution).
+__device__ __forceinline__ void swap_lane_hi_slot_lo(float2 (&v)[16]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    swap_lane_bit4(v[4 * i + 0], v[4 * i + 1]);
+    swap_lane_bit4(v[4 * i + 2], v[4 * i + 3]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    swap_lane_bit5(v[4 * i + 0], v[4 * i + 2]);
+    swap_lane_bit5(v[4 * i + 1], v[4 * i + 3]);
+  }
+}
+
+// Orders this wave's LDS accesses: the compiler may not move loads or stores
+// across it, and one wave's LDS instructions complete in issue order.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+struct WaveFFT1k {
+  static constexpr int S = 1024;
+  static constexpr int NT = 64;
+  static constexpr int RS1 = NT + FDAS_RS1_PAD;  // row stride of the [ka][b] exchange
+  static constexpr int LDS_ELEMS = 16 * RS1;     // one wave's exchange buffer (float2)
+
+  float2 w1[16];                     // W_1024^(lane * ka), in registers
+  const float2* __restrict__ w2;     // LDS copy of W_64^(d * kc) at [kc * 4 + d], d = lane / 16
+
+  // w2_lds: 64 float2 of LDS the calling wave fills (lanes 0..63 copy one entry
+  // each); it is read after the caller's next barrier.
+  __device__ __forceinline__ void load_twiddles(const float2* __restrict__ tw1, const float2* __restrict__ tw2,
+                                                float2* w2_lds, int lane, bool fill) {
+    w1[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) w1[k] = tw1[k * NT + lane];
+    if (fill) w2_lds[lane] = tw2[lane];
+    w2 = w2_lds;
+  }
+
+  // x[a] = tile[64 a + lane] -> spectrum slots (TileFFT<1024>::slot_bin order).
+  __device__ __forceinline__ void forward(float2 (&v)[16], float2* __restrict__ lb, int lane) const {
+    dft16<-1>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], w1[k]);
+    wave_lds_order();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lb[k * RS1 + lane] = v[k];
+    wave_lds_order();
+    const int ka = lane & 15, d = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = lb[ka * RS1 + 4 * c + d];
+    dft16<-1>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], w2[k * 4 + d]);
+    swap_lane_hi_slot_lo(v);
+    dft4<-1, 0, 1>(v);
+    dft4<-1, 4, 1>(v);
+    dft4<-1, 8, 1>(v);
+    dft4<-1, 12, 1>(v);
+  }
+
+  // Spectrum slots -> v[a] = out[64 a + lane] (unnormalised inverse).
+  __device__ __forceinline__ void inverse(float2 (&v)[16], float2* __restrict__ lb, int lane) const {
+    dft4<+1, 0, 1>(v);
+    dft4<+1, 4, 1>(v);
+    dft4<+1, 8, 1>(v);
+    dft4<+1, 12, 1>(v);
+    swap_lane_hi_slot_lo(v);
+    const int ka = lane & 15, d = lane >> 4;
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w2[k * 4 + d]);
+    dft16<+1>(v);
+    wave_lds_order();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lb[ka * RS1 + 4 * c + d] = v[c];
+    wave_lds_order();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lb[k * RS1 + lane];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w1[k]);
+    dft16<+1>(v);
+  }
+};
+
 }  // namespace fdas

@@ -173,16 +173,20 @@ class FDAS:
         thr = np.ascontiguousarray(thresholds, dtype=np.float32)
         return self._ok(self._lib.fdas_perform_harmonic_summing(self._ctx, N.fptr(thr), which, ab))
 
-    def launch(self, input, thresholds, detection_location, detection_power, which, ab) -> bool:
-        """FDAS::launch (FDAS.h:60).  The arrays are kept alive until wait(ab)."""
-        x = _c64(input, (self.get_input_sz(),))
-        thr = np.ascontiguousarray(thresholds, dtype=np.float32)
+    def _check_lists(self, detection_location, detection_power):
+        """The library writes candidate_list_sz entries from each array's first element."""
         n = self.get_candidate_list_sz()
         if detection_location.dtype != np.uint32 or detection_location.size < n or \
                 not detection_location.flags.c_contiguous:
             raise ValueError("detection_location must be a contiguous uint32 array of candidate_list_sz")
         if detection_power.dtype != np.float32 or detection_power.size < n or not detection_power.flags.c_contiguous:
             raise ValueError("detection_power must be a contiguous float32 array of candidate_list_sz")
+
+    def launch(self, input, thresholds, detection_location, detection_power, which, ab) -> bool:
+        """FDAS::launch (FDAS.h:60).  The arrays are kept alive until wait(ab)."""
+        x = _c64(input, (self.get_input_sz(),))
+        thr = np.ascontiguousarray(thresholds, dtype=np.float32)
+        self._check_lists(detection_location, detection_power)
         self._keep[ab] = (x, thr, detection_location, detection_power)
         return self._ok(self._lib.fdas_launch(self._ctx, N.fptr(x.view(np.float32)), N.fptr(thr),
                                               N.uptr(detection_location), N.fptr(detection_power), which, ab))
@@ -190,11 +194,7 @@ class FDAS:
     def launch_resident(self, thresholds, detection_location, detection_power, which, ab) -> bool:
         """launch() on the spectrum already resident in set ab (no host-to-device copy)."""
         thr = np.ascontiguousarray(thresholds, dtype=np.float32)
-        n = self.get_candidate_list_sz()
-        if detection_location.dtype != np.uint32 or detection_location.size < n:
-            raise ValueError("detection_location must be uint32 of candidate_list_sz")
-        if detection_power.dtype != np.float32 or detection_power.size < n:
-            raise ValueError("detection_power must be float32 of candidate_list_sz")
+        self._check_lists(detection_location, detection_power)
         self._keep[ab] = (thr, detection_location, detection_power)
         return self._ok(self._lib.fdas_launch_resident(self._ctx, N.fptr(thr), N.uptr(detection_location),
                                                        N.fptr(detection_power), which, ab))
@@ -227,11 +227,7 @@ class FDAS:
         return self._ok(self._lib.fdas_inject_fop(self._ctx, N.fptr(f), ab))
 
     def retrieve_candidates(self, detection_location, detection_power, ab) -> bool:
-        n = self.get_candidate_list_sz()
-        if detection_location.dtype != np.uint32 or detection_location.size < n:
-            raise ValueError("detection_location must be uint32 of candidate_list_sz")
-        if detection_power.dtype != np.float32 or detection_power.size < n:
-            raise ValueError("detection_power must be float32 of candidate_list_sz")
+        self._check_lists(detection_location, detection_power)
         return self._ok(self._lib.fdas_retrieve_candidates(self._ctx, N.uptr(detection_location),
                                                            N.fptr(detection_power), ab))
 

@@ -12,7 +12,9 @@
  *   - BufferMode (DDR bank placement, FDAS.cpp:188-219) has no GPU meaning and
  *     is accepted and ignored;
  *   - the compile-time GenInfo constants become an optional fdas_config.
- * float2_t has the layout of cl_float2 ({float s[2]}).
+ * Complex arrays may be of any 8-byte trivially copyable float-pair type:
+ * cl_float2 from <CL/cl.hpp>, std::complex<float>, or float2_t below; so the
+ * reference's callers keep their OpenCL types.
  * Header-only; link with libfdas_amd.so.
  */
 #ifndef FDAS_AMD_FDAS_HPP
@@ -22,6 +24,7 @@
 #include <functional>
 #include <ostream>
 #include <string>
+#include <type_traits>
 
 #include "fdas.h"
 
@@ -31,6 +34,20 @@ struct float2_t {
   float s[2];
 };
 static_assert(sizeof(float2_t) == 8, "float2_t must have the layout of cl_float2");
+
+// Interleaved (re, im) floats of a complex array of any 8-byte float-pair type.
+template <typename C2>
+inline const float* complex_floats(const C2* p) {
+  static_assert(sizeof(C2) == 2 * sizeof(float) && std::is_trivially_copyable<C2>::value,
+                "complex elements must be two floats (re, im), e.g. cl_float2 or std::complex<float>");
+  return reinterpret_cast<const float*>(p);
+}
+template <typename C2>
+inline float* complex_floats(C2* p) {
+  static_assert(sizeof(C2) == 2 * sizeof(float) && std::is_trivially_copyable<C2>::value,
+                "complex elements must be two floats (re, im), e.g. cl_float2 or std::complex<float>");
+  return reinterpret_cast<float*>(p);
+}
 
 class FDAS {
  public:
@@ -82,14 +99,16 @@ class FDAS {
     return ok(fdas_create(&cfg, &FDAS::forward_log, this, &ctx));
   }
 
-  bool upload_templates(const float2_t* templates, BufferSet ab) {
-    return ok(fdas_upload_templates(ctx, &templates->s[0], ab));
+  template <typename C2>
+  bool upload_templates(const C2* templates, BufferSet ab) {
+    return ok(fdas_upload_templates(ctx, complex_floats(templates), ab));
   }
   bool upload_thresholds(const float* thresholds, BufferSet ab) {
     return ok(fdas_upload_thresholds(ctx, thresholds, ab));
   }
-  bool perform_input_tiling(const float2_t* input, BufferSet ab) {
-    return ok(fdas_perform_input_tiling(ctx, &input->s[0], ab));
+  template <typename C2>
+  bool perform_input_tiling(const C2* input, BufferSet ab) {
+    return ok(fdas_perform_input_tiling(ctx, complex_floats(input), ab));
   }
   bool perform_ft_convolution(FOPPart which, BufferSet ab) {
     return ok(fdas_perform_ft_convolution(ctx, fdas_part(which), ab));
@@ -97,13 +116,18 @@ class FDAS {
   bool perform_harmonic_summing(const float* thresholds, FOPPart which, BufferSet ab) {
     return ok(fdas_perform_harmonic_summing(ctx, thresholds, fdas_part(which), ab));
   }
-  bool launch(const float2_t* input, const float* thresholds, uint32_t* detection_location, float* detection_power,
+  template <typename C2>
+  bool launch(const C2* input, const float* thresholds, uint32_t* detection_location, float* detection_power,
               FOPPart which, BufferSet ab) {
-    return ok(fdas_launch(ctx, &input->s[0], thresholds, detection_location, detection_power, fdas_part(which), ab));
+    return ok(fdas_launch(ctx, complex_floats(input), thresholds, detection_location, detection_power,
+                          fdas_part(which), ab));
   }
   bool wait(BufferSet ab) { return ok(fdas_wait(ctx, ab)); }
   bool end_pipeline(BufferSet ab) { return ok(fdas_end_pipeline(ctx, ab)); }
-  bool retrieve_tiles(float2_t* tiles, BufferSet ab) { return ok(fdas_retrieve_tiles(ctx, &tiles->s[0], ab)); }
+  template <typename C2>
+  bool retrieve_tiles(C2* tiles, BufferSet ab) {
+    return ok(fdas_retrieve_tiles(ctx, complex_floats(tiles), ab));
+  }
   bool retrieve_FOP(float* fop, BufferSet ab) { return ok(fdas_retrieve_fop(ctx, fop, ab)); }
   bool inject_FOP(const float* fop, BufferSet ab) { return ok(fdas_inject_fop(ctx, fop, ab)); }
   bool retrieve_candidates(uint32_t* detection_location, float* detection_power, BufferSet ab) {

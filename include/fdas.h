@@ -170,7 +170,7 @@ fdas_status fdas_retrieve_candidates_wide(fdas_ctx* ctx, fdas_candidate* out, ui
 
 /* Harmonic plane HP_k (k = 1..H) of the current FOP, T x N floats, rows outside
  * the part left untouched.  Debug / parity hook; the hot path never materialises
- * planes (the reference oracle writes them as hps_ref.npy, hsum.py:334). */
+ * planes (the reference oracle writes them as hps_ref.npy, hsum.py:123). */
 fdas_status fdas_retrieve_harmonic_plane(fdas_ctx* ctx, uint32_t k, float* plane, fdas_part part, int set);
 
 /* FDAS::print_stats / print_events (FDAS.h:74-76, FDAS.cpp:634-686): per-phase
@@ -261,7 +261,7 @@ typedef struct fdas_plan fdas_plan;
 typedef struct fdas_engine_config {
   uint32_t tile;               /* FFT size for every shape; 0: autotuned per shape from 1024/2048/4096 */
   uint32_t rows_per_item;      /* templates per FT-convolution workgroup; 0: autotuned (8/16/32/64) */
-  uint32_t hsum_rows;          /* rows per harmonic-summing workgroup; 0: 11 */
+  uint32_t hsum_rows;          /* rows per harmonic-summing workgroup; 0: autotuned (11/16/22) */
   uint32_t cands_per_harmonic; /* candidates kept per plane, <= 8192 (default 4096) */
   uint32_t spill_per_harmonic; /* detections buffered per plane before the cap
                                   (0, the default: max(262144, T * N / 512) per trial) */
@@ -271,7 +271,9 @@ typedef struct fdas_engine_config {
   uint64_t hbm_budget_bytes;   /* device memory for per-trial state (default 200 GB) */
   uint32_t ftc_workgroups_per_cu; /* FT-convolution workgroups resident per CU (a persistent grid), leaving
                                      room for the harmonic summing that runs beside it; 0: one per item */
-  uint32_t pad_;
+  uint32_t no_fusion;          /* 0 (default): plans run group g's FT convolution and group g-1's harmonic
+                                  summing in one launch, as one fused kernel where the shapes allow (1024-point
+                                  tiles, 8 harmonics, or 16 with N % 8 == 0); 1: separate kernels per group */
 } fdas_engine_config;
 
 /* Request flags. */
@@ -317,7 +319,11 @@ typedef struct fdas_plan_info {
   uint64_t region_bytes;       /* device bytes of the plan's per-trial state */
   float last_ms;               /* device time of the last execution (start to end events) */
   float ftc_ms, hsum_ms;       /* summed FT-convolution / harmonic-summing kernel spans of the last eager
-                                  execution (0 under a graph) */
+                                  execution, outside fused launches (0 under a graph) */
+  uint32_t lagged;             /* 1: the plan runs the lagged schedule (see no_fusion) */
+  uint32_t n_fused;            /* fused launches in the last eager execution */
+  float fused_ms;              /* their summed spans (each: one group's convolution + the previous group's
+                                  summing) */
 } fdas_plan_info;
 
 #define FDAS_PLAN_GRAPH 0x1u   /* capture the plan's launches once as a HIP graph and replay it */
@@ -334,9 +340,34 @@ const char* fdas_engine_last_error(const fdas_engine* eng);
 fdas_status fdas_engine_add_bank(fdas_engine* eng, const float* taps_c64, uint32_t n_templates, uint32_t n_taps,
                                  uint32_t* bank_id);
 
-/* Fixes (autotuning if configured) the tile size and work shapes of
- * (n_bins, bank, part) ahead of traffic; *tile receives the tile size (may be NULL). */
+/* Fixes the tile size and work shapes of (n_bins, bank, part) ahead of traffic,
+ * timing the candidates not fixed by the configuration or the tuning cache on a
+ * seeded noise spectrum: tile x rows per item for the FT convolution, then rows
+ * per summing workgroup for summing + selection; *tile receives the tile size
+ * (may be NULL). */
 fdas_status fdas_engine_prepare(fdas_engine* eng, uint32_t n_bins, uint32_t bank, uint32_t part, uint32_t* tile);
+
+/* The work shape the engine fixed for (n_bins, bank, part) (after
+ * fdas_engine_prepare, a plan or a submission of that shape), with the
+ * transform counts of one trial: forward transforms (one per work item) and
+ * inverse transforms (one per tile and template row). */
+typedef struct fdas_shape_info {
+  uint32_t tile, rows_per_item, hsum_rows, n_row_groups;
+  uint64_t forward_transforms, inverse_transforms;
+  uint64_t stored_outputs;     /* FOP values written: rows x n_bins */
+} fdas_shape_info;
+fdas_status fdas_engine_shape_info(fdas_engine* eng, uint32_t n_bins, uint32_t bank, uint32_t part,
+                                   fdas_shape_info* info);
+
+/* Tuning cache: shapes listed in `path` (written by earlier engines on the same
+ * device) are used without timing, and every shape this engine times is
+ * appended to it.  Key: device, N, bank contents (digest of the taps), part and
+ * cands_per_harmonic; value: tile, rows per item, rows per summing workgroup.
+ * The tile decides the FOP rounding, so engines sharing a cache return
+ * identical lists for identical requests.  A shape first seen while requests
+ * are being served is not timed: it takes the cache's choice, else a cost
+ * model's (fewest transform points per output). */
+fdas_status fdas_engine_tune_cache(fdas_engine* eng, const char* path);
 
 /* Batch: a fixed list of trials, split into groups of at most group_trials; the
  * FT convolution of one group runs on one stream while harmonic summing and

@@ -17,8 +17,17 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
+def isa() -> str:
+    """x86-64-v3 (AVX2 + FMA) where the host CPU has both, else x86-64-v2."""
+    try:
+        flags = Path("/proc/cpuinfo").read_text().split("flags", 2)[1].split("\n", 1)[0].split()
+    except (OSError, IndexError):
+        return "v2"
+    return "v3" if {"avx2", "fma"} <= set(flags) else "v2"
+
+
 def load(precision: str = "f32") -> C.CDLL:
-    path = BUILD / f"libfdas_cpu_{precision}.so"
+    path = BUILD / f"libfdas_cpu_{precision}_{isa()}.so"
     if not path.exists():
         build()
     lib = C.CDLL(str(path))

@@ -4,7 +4,8 @@
  *
  *   fdas_cpu_fop   : FOP[t][f] = |sum_m h_t[m] X[f-m]|^2 (ft_conv.py:199-207), computed
  *                    by overlap-save with tile S and overlap L-1 (ft_conv.py:158-183),
- *                    own iterative radix-2 FFT, one template per OpenMP task.
+ *                    own self-sorting radix-4 FFT on split re/im arrays (vectorised
+ *                    inner loops), one template per OpenMP task.
  *   fdas_cpu_hsum  : HP_1 = FOP, HP_k = HP_{k-1} + FOP[trunc(t/k)][f/k] (hsum.py:91-120),
  *                    strict HP_k > thr_k detections (hsum.py:133-151), capped per plane
  *                    to the `cap` strongest (ties by (row, bin)) and sorted by (row, bin).
@@ -26,117 +27,198 @@
 #define REAL float
 #endif
 
+/* Self-sorting (Stockham) radix-4 FFT on split re/im arrays, decimation in
+ * frequency, with a final radix-2 stage when log2(n) is odd.  Stage t has
+ * stride s = 4^t and length n_t = n / 4^t; its inner loop runs over s
+ * consecutive elements, which the compiler vectorises once s >= 8.  Twiddles
+ * W_n_t^p, ^2p, ^3p of every stage are tabulated in FP64 and rounded once. */
 typedef struct {
-  REAL re, im;
-} cpx;
+  int n, n_stages;  /* radix-4 stages; a radix-2 stage follows when n_stages' lengths end at 2 */
+  REAL* tw;         /* per stage, m = n_t / 4 entries of (w1, w2, w3) as 6 REALs */
+} fft_plan;
 
-static void fft_plan(int n, cpx* tw) {
-  for (int k = 0; k < n / 2; ++k) {
-    const double a = -2.0 * 3.14159265358979323846 * (double)k / (double)n;
-    tw[k].re = (REAL)cos(a);
-    tw[k].im = (REAL)sin(a);
+static int fft_plan_make(fft_plan* pl, int n) {
+  pl->n = n;
+  pl->n_stages = 0;
+  size_t total = 0;
+  for (int nt = n; nt >= 4; nt /= 4) {
+    total += (size_t)(nt / 4) * 6;
+    pl->n_stages++;
   }
-}
-
-/* In-place iterative radix-2 DFT; dir = -1 forward, +1 inverse (unnormalised). */
-static void fft_run(cpx* a, int n, const cpx* tw, int dir) {
-  for (int i = 1, j = 0; i < n; ++i) {
-    int bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) {
-      cpx t = a[i];
-      a[i] = a[j];
-      a[j] = t;
-    }
-  }
-  for (int len = 2; len <= n; len <<= 1) {
-    const int step = n / len;
-    for (int i = 0; i < n; i += len) {
-      for (int k = 0; k < len / 2; ++k) {
-        cpx w = tw[k * step];
-        if (dir > 0) w.im = -w.im;
-        cpx u = a[i + k], v = a[i + k + len / 2];
-        const REAL vr = v.re * w.re - v.im * w.im, vi = v.re * w.im + v.im * w.re;
-        a[i + k].re = u.re + vr;
-        a[i + k].im = u.im + vi;
-        a[i + k + len / 2].re = u.re - vr;
-        a[i + k + len / 2].im = u.im - vi;
+  pl->tw = (REAL*)malloc(sizeof(REAL) * (total ? total : 1));
+  if (!pl->tw) return -2;
+  REAL* w = pl->tw;
+  for (int nt = n; nt >= 4; nt /= 4) {
+    const int m = nt / 4;
+    for (int p = 0; p < m; ++p) {
+      for (int e = 1; e <= 3; ++e) {
+        const double a = -2.0 * 3.14159265358979323846 * (double)(((long)e * p) % nt) / (double)nt;
+        w[6 * p + 2 * (e - 1)] = (REAL)cos(a);
+        w[6 * p + 2 * (e - 1) + 1] = (REAL)sin(a);
       }
     }
+    w += (size_t)m * 6;
+  }
+  return 0;
+}
+
+static void fft_plan_free(fft_plan* pl) { free(pl->tw); }
+
+/* Forward DFT (e^{-2 pi i f n / N}, unnormalised) of (xr, xi) in place; (yr, yi) is work space. */
+static void fft_forward(const fft_plan* pl, REAL* xr, REAL* xi, REAL* yr, REAL* yi) {
+  const int n = pl->n;
+  REAL *ar = xr, *ai = xi, *br = yr, *bi = yi;
+  const REAL* w = pl->tw;
+  int s = 1, nt = n;
+  for (; nt >= 4; nt /= 4, s *= 4) {
+    const int m = nt / 4;
+    for (int p = 0; p < m; ++p) {
+      const REAL w1r = w[6 * p], w1i = w[6 * p + 1], w2r = w[6 * p + 2], w2i = w[6 * p + 3];
+      const REAL w3r = w[6 * p + 4], w3i = w[6 * p + 5];
+      const REAL *a0r = ar + (size_t)s * p, *a0i = ai + (size_t)s * p;
+      const REAL *a1r = a0r + (size_t)s * m, *a1i = a0i + (size_t)s * m;
+      const REAL *a2r = a1r + (size_t)s * m, *a2i = a1i + (size_t)s * m;
+      const REAL *a3r = a2r + (size_t)s * m, *a3i = a2i + (size_t)s * m;
+      REAL *y0r = br + (size_t)s * 4 * p, *y0i = bi + (size_t)s * 4 * p;
+      REAL *y1r = y0r + s, *y1i = y0i + s, *y2r = y1r + s, *y2i = y1i + s, *y3r = y2r + s, *y3i = y2i + s;
+      for (int q = 0; q < s; ++q) {
+        const REAL apcr = a0r[q] + a2r[q], apci = a0i[q] + a2i[q];
+        const REAL amcr = a0r[q] - a2r[q], amci = a0i[q] - a2i[q];
+        const REAL bpdr = a1r[q] + a3r[q], bpdi = a1i[q] + a3i[q];
+        const REAL bmdr = a1r[q] - a3r[q], bmdi = a1i[q] - a3i[q];
+        /* j*(b - d) with j = i */
+        const REAL jr = -bmdi, ji = bmdr;
+        y0r[q] = apcr + bpdr;
+        y0i[q] = apci + bpdi;
+        const REAL t1r = amcr - jr, t1i = amci - ji;
+        y1r[q] = t1r * w1r - t1i * w1i;
+        y1i[q] = t1r * w1i + t1i * w1r;
+        const REAL t2r = apcr - bpdr, t2i = apci - bpdi;
+        y2r[q] = t2r * w2r - t2i * w2i;
+        y2i[q] = t2r * w2i + t2i * w2r;
+        const REAL t3r = amcr + jr, t3i = amci + ji;
+        y3r[q] = t3r * w3r - t3i * w3i;
+        y3i[q] = t3r * w3i + t3i * w3r;
+      }
+    }
+    w += (size_t)m * 6;
+    REAL* t;
+    t = ar, ar = br, br = t;
+    t = ai, ai = bi, bi = t;
+  }
+  if (nt == 2) { /* last radix-2 stage, no twiddle */
+    for (int q = 0; q < s; ++q) {
+      const REAL a0r = ar[q], a0i = ai[q], a1r = ar[q + s], a1i = ai[q + s];
+      br[q] = a0r + a1r;
+      bi[q] = a0i + a1i;
+      br[q + s] = a0r - a1r;
+      bi[q + s] = a0i - a1i;
+    }
+    REAL* t;
+    t = ar, ar = br, br = t;
+    t = ai, ai = bi, bi = t;
+  }
+  if (ar != xr) {
+    memcpy(xr, ar, sizeof(REAL) * (size_t)n);
+    memcpy(xi, ai, sizeof(REAL) * (size_t)n);
   }
 }
 
 int fdas_cpu_fop(const float* x_c64, int n, const float* taps_c64, int t_count, int taps, int tile, float* fop,
                  int n_threads) {
   if (!x_c64 || !taps_c64 || !fop || n <= 0 || t_count <= 0 || taps <= 0 || tile < 2 * taps) return -1;
+  if (tile & (tile - 1)) return -1;
   const int o = taps - 1, p = tile - o;
   const int n_tiles = (n + p - 1) / p;
-  cpx* tw = (cpx*)malloc(sizeof(cpx) * (size_t)tile / 2);
-  cpx* xt = (cpx*)malloc(sizeof(cpx) * (size_t)tile * (size_t)n_tiles);
-  if (!tw || !xt) {
-    free(tw);
-    free(xt);
+  fft_plan pl;
+  if (fft_plan_make(&pl, tile)) return -2;
+  /* transformed tiles, split: re then im per tile */
+  REAL* xt = (REAL*)malloc(sizeof(REAL) * 2 * (size_t)tile * (size_t)n_tiles);
+  if (!xt) {
+    fft_plan_free(&pl);
     return -2;
   }
-  fft_plan(tile, tw);
 #ifdef _OPENMP
   if (n_threads > 0) omp_set_num_threads(n_threads);
 #endif
-  /* forward transform of every tile: tile j holds X[j*P - O + s] (ft_conv.py:164-180) */
-#pragma omp parallel for schedule(static)
-  for (int j = 0; j < n_tiles; ++j) {
-    cpx* t = xt + (size_t)j * tile;
-    for (int s = 0; s < tile; ++s) {
-      const long idx = (long)j * p - o + s;
-      if (idx >= 0 && idx < n) {
-        t[s].re = (REAL)x_c64[2 * idx];
-        t[s].im = (REAL)x_c64[2 * idx + 1];
-      } else {
-        t[s].re = t[s].im = 0;
-      }
-    }
-    fft_run(t, tile, tw, -1);
-  }
   int err = 0;
+  /* forward transform of every tile: tile j holds X[j*P - O + s] (ft_conv.py:164-180) */
 #pragma omp parallel
   {
-    cpx* h = (cpx*)malloc(sizeof(cpx) * (size_t)tile);
-    cpx* y = (cpx*)malloc(sizeof(cpx) * (size_t)tile);
-    if (!h || !y) {
+    REAL* wk = (REAL*)malloc(sizeof(REAL) * 2 * (size_t)tile);
+    if (!wk) {
+#pragma omp atomic write
+      err = -2;
+    }
+#pragma omp for schedule(static)
+    for (int j = 0; j < n_tiles; ++j) {
+      if (!wk) continue;
+      REAL* tr = xt + (size_t)j * 2 * tile;
+      REAL* ti = tr + tile;
+      for (int s = 0; s < tile; ++s) {
+        const long idx = (long)j * p - o + s;
+        if (idx >= 0 && idx < n) {
+          tr[s] = (REAL)x_c64[2 * idx];
+          ti[s] = (REAL)x_c64[2 * idx + 1];
+        } else {
+          tr[s] = ti[s] = 0;
+        }
+      }
+      fft_forward(&pl, tr, ti, wk, wk + tile);
+    }
+    free(wk);
+  }
+  if (err) {
+    free(xt);
+    fft_plan_free(&pl);
+    return err;
+  }
+#pragma omp parallel
+  {
+    REAL* buf = (REAL*)malloc(sizeof(REAL) * 6 * (size_t)tile); /* h (split), y (split), work (split) */
+    if (!buf) {
 #pragma omp atomic write
       err = -2;
     }
 #pragma omp for schedule(dynamic, 1)
     for (int r = 0; r < t_count; ++r) {
-      if (!h || !y) continue;
+      if (!buf) continue;
+      REAL *hr = buf, *hi = buf + tile, *yr = buf + 2 * tile, *yi = buf + 3 * tile, *wr = buf + 4 * tile,
+           *wi = buf + 5 * tile;
       for (int s = 0; s < tile; ++s) {
-        h[s].re = h[s].im = 0;
+        hr[s] = hi[s] = 0;
         if (s < taps) {
-          h[s].re = (REAL)taps_c64[2 * ((size_t)r * taps + s)];
-          h[s].im = (REAL)taps_c64[2 * ((size_t)r * taps + s) + 1];
+          hr[s] = (REAL)taps_c64[2 * ((size_t)r * taps + s)];
+          hi[s] = (REAL)taps_c64[2 * ((size_t)r * taps + s) + 1];
         }
       }
-      fft_run(h, tile, tw, -1);
+      fft_forward(&pl, hr, hi, wr, wi);
+      /* fold the 1/S of the inverse into the template spectrum */
       const REAL inv = (REAL)1 / (REAL)tile;
+      for (int s = 0; s < tile; ++s) {
+        hr[s] *= inv;
+        hi[s] *= inv;
+      }
       for (int j = 0; j < n_tiles; ++j) {
-        const cpx* t = xt + (size_t)j * tile;
+        const REAL* tr = xt + (size_t)j * 2 * tile;
+        const REAL* ti = tr + tile;
+        /* inverse DFT as conj(DFT(conj(.))): conjugated product in, conjugate ignored out (|y|^2) */
         for (int s = 0; s < tile; ++s) {
-          y[s].re = (t[s].re * h[s].re - t[s].im * h[s].im) * inv;
-          y[s].im = (t[s].re * h[s].im + t[s].im * h[s].re) * inv;
+          yr[s] = tr[s] * hr[s] - ti[s] * hi[s];
+          yi[s] = -(tr[s] * hi[s] + ti[s] * hr[s]);
         }
-        fft_run(y, tile, tw, +1);
-        for (int s = o; s < tile; ++s) {
-          const long f = (long)j * p + s - o;
-          if (f < n) fop[(size_t)r * n + f] = (float)(y[s].re * y[s].re + y[s].im * y[s].im);
-        }
+        fft_forward(&pl, yr, yi, wr, wi);
+        /* sample s >= O of tile j is FOP bin j*P + s - O; bins past N are dropped */
+        const long f0 = (long)j * p - o;
+        const int s_end = f0 + tile > n ? (int)(n - f0) : tile;
+        float* row = fop + (size_t)r * n;
+        for (int s = o; s < s_end; ++s) row[f0 + s] = (float)(yr[s] * yr[s] + yi[s] * yi[s]);
       }
     }
-    free(h);
-    free(y);
+    free(buf);
   }
-  free(tw);
   free(xt);
+  fft_plan_free(&pl);
   return err;
 }
 
@@ -176,48 +258,64 @@ int fdas_cpu_hsum(const float* fop, int t_count, int n, int n_harm, const float*
     free(lens);
     return -2;
   }
+  /* Rows in parallel; each row in chunks of CHUNK bins so the partial sums stay
+   * in cache while the planes are added (the own row and the sub-rows are read
+   * once per chunk).  Detections of a (row, plane) are appended in bin order. */
+  enum { CHUNK = 8192 };
 #pragma omp parallel
   {
-    REAL* hp = (REAL*)malloc(sizeof(REAL) * (size_t)n);
+    REAL* hp = (REAL*)malloc(sizeof(REAL) * CHUNK);
+    size_t* capl = (size_t*)calloc((size_t)n_harm, sizeof(size_t));
 #pragma omp for schedule(dynamic, 1)
     for (int r = 0; r < t_count; ++r) {
-      if (!hp) {
+      if (!hp || !capl) {
 #pragma omp atomic write
         err = -2;
         continue;
       }
       const int t = r - c;
-      for (int k = 1; k <= n_harm; ++k) {
-        const float* src = fop + (size_t)(c + t / k) * n; /* C division truncates toward zero */
-        if (k == 1)
-          for (int f = 0; f < n; ++f) hp[f] = (REAL)src[f];
-        else
-          for (int f = 0; f < n; ++f) hp[f] += (REAL)src[f / k];
-        const REAL th = (REAL)thr[k - 1];
-        size_t cnt = 0, capl = 0;
-        cand* l = NULL;
-        for (int f = 0; f < n; ++f) {
-          if (hp[f] > th) {
-            if (cnt == capl) {
-              capl = capl ? 2 * capl : 64;
-              cand* nl = (cand*)realloc(l, capl * sizeof(cand));
+      cand** lr = lists + (size_t)r * n_harm;
+      size_t* nr = lens + (size_t)r * n_harm;
+      for (int k = 0; k < n_harm; ++k) capl[k] = 0;
+      for (int f0 = 0; f0 < n; f0 += CHUNK) {
+        const int m = n - f0 < CHUNK ? n - f0 : CHUNK;
+        for (int k = 1; k <= n_harm; ++k) {
+          const float* src = fop + (size_t)(c + t / k) * n; /* C division truncates toward zero */
+          if (k == 1) {
+            for (int i = 0; i < m; ++i) hp[i] = (REAL)src[f0 + i];
+          } else {
+            /* bins f .. f+k-1 with f a multiple of k share sub-harmonic bin f/k */
+            int i = 0;
+            for (; i < m && (f0 + i) % k; ++i) hp[i] += (REAL)src[(f0 + i) / k];
+            for (; i + k <= m; i += k) {
+              const REAL v = (REAL)src[(f0 + i) / k];
+              for (int u = 0; u < k; ++u) hp[i + u] += v;
+            }
+            for (; i < m; ++i) hp[i] += (REAL)src[(f0 + i) / k];
+          }
+          const REAL th = (REAL)thr[k - 1];
+          for (int i = 0; i < m; ++i) {
+            if (!(hp[i] > th)) continue;
+            size_t* cnt = &nr[k - 1];
+            if (*cnt == capl[k - 1]) {
+              capl[k - 1] = capl[k - 1] ? 2 * capl[k - 1] : 64;
+              cand* nl = (cand*)realloc(lr[k - 1], capl[k - 1] * sizeof(cand));
               if (!nl) {
 #pragma omp atomic write
                 err = -2;
                 break;
               }
-              l = nl;
+              lr[k - 1] = nl;
             }
-            l[cnt].pwr = (float)hp[f];
-            l[cnt].key = ((uint32_t)r << 22) | (uint32_t)f;
-            ++cnt;
+            lr[k - 1][*cnt].pwr = (float)hp[i];
+            lr[k - 1][*cnt].key = ((uint32_t)r << 22) | (uint32_t)(f0 + i);
+            ++*cnt;
           }
         }
-        lists[(size_t)r * n_harm + (k - 1)] = l;
-        lens[(size_t)r * n_harm + (k - 1)] = cnt;
       }
     }
     free(hp);
+    free(capl);
   }
   for (int k = 0; k < n_harm && !err; ++k) {
     size_t total = 0;

@@ -311,7 +311,7 @@ def harmonic_planes(fop: np.ndarray, n_harmonics: int, rows=None, dtype=np.float
 
 
 def harmonic_planes_loop(fop: np.ndarray, n_harmonics: int) -> np.ndarray:
-    """Literal per-element restatement of hsum.py:306-331 (float32, tiny inputs only).
+    """Literal per-element restatement of hsum.py:93-120 (float32, tiny inputs only).
 
     Keeps the reference's ceil / floor branches on the sign of the template
     number instead of the closed-form truncation used by harmonic_planes.
@@ -330,7 +330,7 @@ def harmonic_planes_loop(fop: np.ndarray, n_harmonics: int) -> np.ndarray:
                 sp = int(np.ceil(num / k)) + c
             elif num > 0:
                 sp = int(np.floor(num / k)) + c
-            else:  # the reference keeps the previous value here, which is c (hsum.py:324-325)
+            else:  # the reference keeps the previous value here, which is c (hsum.py:113-114)
                 pass
             for freq in range(n):
                 hps[h, tmpl, freq] = np.float32(hps[h - 1, tmpl, freq] + fop[sp, freq // k])

@@ -40,7 +40,7 @@ def small_case():
 
 
 def hsum_literal_case():
-    """HSUM fixture from the literal per-element loop (hsum.py:306-331), float32, T = 9, N = 50."""
+    """HSUM fixture from the literal per-element loop (hsum.py:93-120), float32, T = 9, N = 50."""
     rng = np.random.default_rng(7)
     fop = rng.exponential(1.0, size=(9, 50)).astype(np.float32)
     hps = O.harmonic_planes_loop(fop, 8)

@@ -94,7 +94,8 @@ def test_drift_bank_rejects_too_long_template():
 
 @pytest.mark.parametrize("field,value", [("tile", 3000), ("cands_per_harmonic", 0), ("cands_per_harmonic", 8193),
                                          ("groups_in_flight", 1), ("groups_in_flight", 9), ("group_trials", 257),
-                                         ("spill_per_harmonic", 100), ("rows_per_item", 1000)])
+                                         ("spill_per_harmonic", 100), ("rows_per_item", 1000),
+                                         ("no_fusion", 2)])
 def test_invalid_engine_config_rejected_before_device(field, value):
     lib = N.load()
     cfg = N.EngineConfig()
@@ -125,4 +126,30 @@ def test_engine_structures_match_header():
     assert C.sizeof(N.EngineConfig) == 48
     assert C.sizeof(N.Request) == 8 + 6 * 4 + 16 * 4 + 8
     assert C.sizeof(N.Result) == 8 + 8 + 8 + 4 + 4 + 64 + 64 + 4 + 4 + 8
-    assert C.sizeof(N.PlanInfo) == 16 + 8 + 12 + 4
+    assert C.sizeof(N.PlanInfo) == 16 + 8 + 12 + 12
+    assert C.sizeof(N.ShapeInfo) == 16 + 3 * 8
+
+
+def test_engine_tuning_entry_points_reject_null():
+    lib = N.load()
+    assert lib.fdas_engine_tune_cache(None, b"x") == 1
+    assert lib.fdas_engine_shape_info(None, 1, 0, 2, None) == 1
+
+
+@pytest.mark.parametrize("method", ["retrieve_candidates", "launch_resident"])
+def test_candidate_arrays_must_be_contiguous(method):
+    """The library writes candidate_list_sz entries from each array's first element, so a
+    strided or reversed view is refused before any call reaches the device."""
+    f = fdas_amd.FDAS(log=lambda lvl, msg: None)
+    loc = np.zeros(64, np.uint32)[::-1]
+    pwr = np.zeros(64, np.float32)
+    with pytest.raises(ValueError, match="contiguous"):
+        if method == "retrieve_candidates":
+            f.retrieve_candidates(loc, pwr, 0)
+        else:
+            f.launch_resident(np.ones(8, np.float32), loc, pwr, 2, 0)
+    with pytest.raises(ValueError, match="contiguous"):
+        if method == "retrieve_candidates":
+            f.retrieve_candidates(np.zeros(64, np.uint32), np.zeros(128, np.float32)[::2], 0)
+        else:
+            f.launch_resident(np.ones(8, np.float32), np.zeros(64, np.uint32), np.zeros(128, np.float32)[::2], 2, 0)

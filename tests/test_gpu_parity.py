@@ -7,7 +7,7 @@ Bars (from the reference's tests, tightened):
     within 1e-4 relative of the plane's cut (its threshold, or the power of the
     last kept entry when the cap binds); powers within 1e-5 of the largest;
   * harmonic summing fed the oracle's own float32 FOP: bit-exact planes and lists
-    (same FP32 additions in the same order as hsum.py:331).
+    (same FP32 additions in the same order as hsum.py:93-120).
 """
 
 import numpy as np
