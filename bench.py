@@ -280,6 +280,7 @@ def batch_workload(args):
 
 
 def batched_form(args, eng, bank85, workload):
+    import torch
     from fdas_amd.engine import Trial
     trials = [Trial(x, bank85, thr, 2, tag=1000 + i) for i, (x, thr) in enumerate(workload)]
     tile = eng.prepare(N_METRIC, bank85, 2)

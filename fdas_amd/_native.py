@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfdas_amd.so"
 
 FDAS_OK = 0
 STATUS_NAMES = {0: "FDAS_OK", 1: "FDAS_EINVAL", 2: "FDAS_ENOMEM", 3: "FDAS_EDEVICE", 4: "FDAS_EUNSUPPORTED",
-                5: "FDAS_EBUSY", 6: "FDAS_ESTATE", 7: "FDAS_EOVERFLOW"}
+                5: "FDAS_EBUSY", 6: "FDAS_ESTATE", 7: "FDAS_EOVERFLOW", 8: "FDAS_ECANCELED"}
 
 FLAG_REFERENCE_FFT_ORDER = 0x1
 FLAG_SYNC_PIPELINE = 0x2
@@ -200,6 +200,7 @@ SIGNATURES = {
     "fdas_engine_submit": (_I, [_P, C.POINTER(Request), C.POINTER(Result)]),
     "fdas_engine_poll": (_I, [_P, C.POINTER(C.POINTER(Result)), _u32, _U, C.c_int32]),
     "fdas_engine_get_stats": (_I, [_P, C.POINTER(EngineStats)]),
+    "fdas_engine_cancel": (_I, [_P, C.c_uint64, _U]),
 }
 
 _lib = None

@@ -416,6 +416,7 @@ extern "C" const char* fdas_status_string(fdas_status s) {
     case FDAS_EBUSY: return "busy";
     case FDAS_ESTATE: return "call out of order";
     case FDAS_EOVERFLOW: return "candidate spill overflow";
+    case FDAS_ECANCELED: return "request canceled";
   }
   return "unknown status";
 }

@@ -1052,7 +1052,9 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight, bool& mem
       const bool never = t.lay.total > e->arena->budget() || (picked.empty() && !others_in_flight && e->live_plans == 0);
       if (never) {
         {
+          // fdas_engine_cancel may have withdrawn it meanwhile
           std::lock_guard<std::mutex> lk(e->q_mu);
+          if (e->queue.empty() || e->queue.front().res != t.res) continue;
           e->queue.pop_front();
           e->stats.queued -= 1;
         }
@@ -1067,6 +1069,10 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight, bool& mem
     }
     {
       std::lock_guard<std::mutex> lk(e->q_mu);
+      if (e->queue.empty() || e->queue.front().res != t.res) {  // withdrawn by fdas_engine_cancel meanwhile
+        e->arena->free(mem);
+        continue;
+      }
       e->queue.pop_front();
       e->stats.queued -= 1;
       e->stats.in_flight += 1;
@@ -1780,6 +1786,36 @@ fdas_status fdas_engine_poll(fdas_engine* e, fdas_result** done, uint32_t max, u
     done[(*n_done)++] = e->done.front();
     e->done.pop_front();
   }
+  return FDAS_OK;
+}
+
+fdas_status fdas_engine_cancel(fdas_engine* e, uint64_t tag, uint32_t* n_canceled) {
+  if (!e) return FDAS_EINVAL;
+  uint32_t n = 0;
+  {
+    std::lock_guard<std::mutex> lk(e->q_mu);
+    for (auto it = e->queue.begin(); it != e->queue.end();) {
+      if (it->req.tag != tag) {
+        ++it;
+        continue;
+      }
+      fdas_result* r = it->res;
+      r->tag = tag;
+      r->status = FDAS_ECANCELED;
+      r->n_harmonics = it->req.n_harmonics;
+      r->tile = it->shape ? it->shape->tile : 0;
+      r->latency_ms = now_ms() - it->t_submit;
+      std::memset(r->n_detected, 0, sizeof(r->n_detected));
+      std::memset(r->n_kept, 0, sizeof(r->n_kept));
+      e->done.push_back(r);
+      e->stats.queued -= 1;
+      e->stats.completed += 1;
+      it = e->queue.erase(it);
+      ++n;
+    }
+    if (n) e->done_cv.notify_all();
+  }
+  if (n_canceled) *n_canceled = n;
   return FDAS_OK;
 }
 

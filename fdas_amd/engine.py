@@ -297,6 +297,13 @@ class Engine:
             done.append(out.result(self.cap))
         return done
 
+    def cancel(self, tag):
+        """Withdraw queued requests with this tag (fdas_engine_cancel); returns how many.  They complete
+        with status FDAS_ECANCELED and come back from poll()."""
+        n = C.c_uint32()
+        self._check(self._lib.fdas_engine_cancel(self._h, C.c_uint64(tag), C.byref(n)))
+        return n.value
+
     def drain(self):
         out = []
         while self._pending:

@@ -36,7 +36,8 @@ typedef enum fdas_status {
   FDAS_EUNSUPPORTED = 4,  /* valid request outside what this build or format supports */
   FDAS_EBUSY = 5,         /* resource in use (e.g. serving queue full) */
   FDAS_ESTATE = 6,        /* call out of order (e.g. convolution before templates) */
-  FDAS_EOVERFLOW = 7      /* detections exceeded the spill capacity; lists are incomplete */
+  FDAS_EOVERFLOW = 7,     /* detections exceeded the spill capacity; lists are incomplete */
+  FDAS_ECANCELED = 8      /* a served request was withdrawn before it reached the device */
 } fdas_status;
 
 /* FDAS::FOPPart (FDAS.h:38): t in [-c, 0], [0, c] or [-c, c]. */
@@ -394,6 +395,12 @@ fdas_status fdas_engine_submit(fdas_engine* eng, const fdas_request* req, fdas_r
 fdas_status fdas_engine_poll(fdas_engine* eng, fdas_result** done, uint32_t max, uint32_t* n_done,
                              int32_t timeout_ms);
 fdas_status fdas_engine_get_stats(fdas_engine* eng, fdas_engine_stats* stats);
+/* Withdraws the submitted requests with this tag that are still queued (not yet
+ * admitted to the device): each completes at once with status FDAS_ECANCELED
+ * and is returned by fdas_engine_poll like any other result.  Requests already
+ * on the device run to completion.  *n_canceled (may be NULL) receives the
+ * number withdrawn. */
+fdas_status fdas_engine_cancel(fdas_engine* eng, uint64_t tag, uint32_t* n_canceled);
 
 #ifdef __cplusplus
 }

@@ -153,3 +153,9 @@ def test_candidate_arrays_must_be_contiguous(method):
             f.retrieve_candidates(np.zeros(64, np.uint32), np.zeros(128, np.float32)[::2], 0)
         else:
             f.launch_resident(np.ones(8, np.float32), np.zeros(64, np.uint32), np.zeros(128, np.float32)[::2], 2, 0)
+
+
+def test_engine_cancel_rejects_null_engine():
+    lib = N.load()
+    assert lib.fdas_engine_cancel(None, 1, None) == 1
+    assert lib.fdas_status_string(8) == b"request canceled"

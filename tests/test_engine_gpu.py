@@ -309,3 +309,30 @@ def test_fused_schedule_equals_separate_kernels(banks):
         _, _, sel = O.run(x, banks[T], thr, len(thr), cap, "all")
         r = out[True][i]
         assert_lists_match(planes_of(r, T), r.n_detected, sel, thr, cap, T // 2)
+
+
+def test_served_requests_can_be_canceled_while_queued(banks):
+    """Requests withdrawn while still queued complete at once with FDAS_ECANCELED; the others
+    complete normally with the lists they give alone."""
+    cap = 64
+    with Engine(tile=2048, cands_per_harmonic=cap, group_trials=2, groups_in_flight=2, log=quiet) as eng:
+        b = eng.add_bank(banks[85])
+        n = 1 << 18
+        eng.prepare(n, b, 2)
+        x = spectrum(n, 811)
+        thr = synth.noise_thresholds(x, 8)
+        alone = eng.run([Trial(x, b, thr, 2, tag=0)])[0]
+        outs = [eng.submit(Trial(x, b, thr, 2, tag=100 + i)) for i in range(24)]
+        withdrawn = eng.cancel(123) + eng.cancel(122)  # the last two submitted
+        assert eng.cancel(99999) == 0
+        done = {r.tag: r for r in eng.drain()}
+        assert len(done) == 24
+        canceled = [t for t, r in done.items() if r.status == 8]
+        assert len(canceled) == withdrawn and set(canceled) <= {122, 123}
+        for t, r in done.items():
+            if r.status != 8:
+                assert r.status == 0
+                assert np.array_equal(r.n_detected, alone.n_detected)
+                for a, c in zip(r.keys, alone.keys):
+                    assert np.array_equal(a, c)
+        assert eng.stats()["completed"] == eng.stats()["submitted"]
