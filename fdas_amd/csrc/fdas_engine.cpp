@@ -330,7 +330,7 @@ struct fdas_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = false, launched = false, eager_timed = false;
-  bool fused = false;  // lagged schedule: launch g runs group g's convolution with group g-1's summing
+  bool fused_kernel = false;  // lagged launches as one fused kernel where the shapes allow
   uint32_t n_launches = 0;
 };
 
@@ -1241,7 +1241,7 @@ fdas_status fdas_engine_create(const fdas_engine_config* cfg_in, fdas_log_fn log
     return fail(e.get(), FDAS_EINVAL, "tile %u is not 1024, 2048 or 4096", c.tile);
   if (c.rows_per_item > 256) return fail(e.get(), FDAS_EINVAL, "rows_per_item %u above 256", c.rows_per_item);
   if (c.hsum_rows > 256) return fail(e.get(), FDAS_EINVAL, "hsum_rows %u above 256", c.hsum_rows);
-  if (c.no_fusion > 1) return fail(e.get(), FDAS_EINVAL, "no_fusion %u is not 0 or 1", c.no_fusion);
+  if (c.fused_kernel > 1) return fail(e.get(), FDAS_EINVAL, "fused_kernel %u is not 0 or 1", c.fused_kernel);
   if (c.ftc_workgroups_per_cu > 64)
     return fail(e.get(), FDAS_EINVAL, "ftc_workgroups_per_cu %u above 64", c.ftc_workgroups_per_cu);
   if (c.cands_per_harmonic < 1 || c.cands_per_harmonic > kSelectMaxCap)
@@ -1421,7 +1421,7 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
   auto p = std::make_unique<fdas_plan>();
   p->eng = e;
   p->use_graph = (flags & FDAS_PLAN_GRAPH) != 0;
-  p->fused = e->cfg.no_fusion == 0;
+  p->fused_kernel = e->cfg.fused_kernel != 0;
   p->trials.resize(n);
   for (uint32_t i = 0; i < n; ++i) {
     fdas_status st = make_trial(e, reqs[i], false, &p->trials[i]);
@@ -1570,7 +1570,7 @@ static fdas_status plan_enqueue_lagged(fdas_plan* p, bool capturing) {
     }
     hipEvent_t t0 = cur ? cur->ev_ftc0 : prev->ev_hsum0, t1 = cur ? cur->ev_ftc_t1 : prev->ev_hsum1;
     if (!capturing) EHIP(e, hipEventRecord(t0, A));
-    const bool fuse = cur && prev && cur->ftc.size() == 1 && prev->hsum.size() == 1 &&
+    const bool fuse = p->fused_kernel && cur && prev && cur->ftc.size() == 1 && prev->hsum.size() == 1 &&
                       cur->ftc[0].tile == 1024 &&
                       fused_supported(1024, prev->hsum[0].H, prev->hsum[0].vec);
     if (cur) cur->launch_fused = fuse;
@@ -1609,21 +1609,7 @@ static fdas_status plan_enqueue_lagged(fdas_plan* p, bool capturing) {
   return FDAS_OK;
 }
 
-static fdas_status plan_enqueue(fdas_plan* p, bool capturing) {
-  fdas_engine* e = p->eng;
-  if (p->fused) return plan_enqueue_lagged(p, capturing);
-  const uint32_t R = uint32_t(p->regions.size());
-  for (uint32_t g = 0; g < p->groups.size(); ++g) {
-    std::vector<Trial*> ptrs;
-    for (uint32_t i : p->groups[g]) ptrs.push_back(&p->trials[i]);
-    const fdas_status st = enqueue_group(e, p->bufs[g], ptrs, g >= R ? p->bufs[g - R].ev_done : nullptr,
-                                         g > 0 ? p->bufs[g - 1].ev_hsum_done : nullptr, !capturing);
-    if (st != FDAS_OK) return st;
-  }
-  // Join the summing stream back into the convolution stream.
-  EHIP(e, hipStreamWaitEvent(e->s_ftc, p->bufs.back().ev_done, 0));
-  return FDAS_OK;
-}
+static fdas_status plan_enqueue(fdas_plan* p, bool capturing) { return plan_enqueue_lagged(p, capturing); }
 
 fdas_status fdas_plan_launch(fdas_plan* p) {
   if (!p) return FDAS_EINVAL;
@@ -1698,13 +1684,13 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p->ev_start, p->ev_end) == hipSuccess) out->last_ms = ms;
     (void)hipGetLastError();
-    out->lagged = p->fused ? 1u : 0u;
+    out->lagged = 1u;
     for (size_t gi = 0; gi < p->bufs.size(); ++gi) {
       if (!p->eager_timed) break;
       const GroupBuf& g = p->bufs[gi];
       float a = 0.f, b = 0.f;
       if (hipEventElapsedTime(&a, g.ev_ftc0, g.ev_ftc_t1) == hipSuccess) {
-        if (p->fused && g.launch_fused) {
+        if (g.launch_fused) {
           out->fused_ms += a;
           out->n_fused += 1;
         } else {
@@ -1712,7 +1698,7 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
         }
       }
       (void)hipGetLastError();
-      const bool summed_here = !p->fused || gi + 1 == p->bufs.size();  // lagged: the last group's own launch
+      const bool summed_here = gi + 1 == p->bufs.size();  // the last group is summed in a launch of its own
       if (summed_here && hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
       (void)hipGetLastError();
     }

@@ -12,6 +12,8 @@
 //                 (detect.cl.mako:106-115) with a deterministic rule.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fdas_kernels.h"
 #include "fft_tile.h"
 
@@ -111,27 +113,26 @@ __global__ __launch_bounds__(S / 16) void ftconv_kernel(const FtcJob* __restrict
   }
 }
 
-// 1024-point tiles, four waves per workgroup sharing one tile.  Wave 0 loads
-// the tile and forward-transforms it into LDS (xs); then every wave convolves
-// its share of the item's template rows (rows w, w + 4, ...), each row reading
-// the tile spectrum back from LDS.  Keeping the spectrum in LDS instead of
-// registers, and exchanging the F2 <-> F3 lane bits with permlane swaps
-// instead of LDS, brings the kernel to three waves per SIMD.
+// 1024-point work items for four-wave workgroups (the fused launch below):
+// wave 0 loads the tile and forward-transforms it into LDS (xs); then every
+// wave convolves its share of the item's template rows (rows w, w + 4, ...),
+// each row reading the tile spectrum back from LDS, so the workgroup runs at
+// three waves per SIMD.
 constexpr int kFtc1kWaves = 4;
 
 // LDS of one 1024-point work item, carved from a caller's buffer.
 struct Ftc1kLds {
-  float2 xs[16 * WaveFFT1k::NT];                   // the tile's spectrum, slot-major
-  float2 lbuf[kFtc1kWaves][WaveFFT1k::LDS_ELEMS];  // each wave's exchange buffer
+  float2 xs[16 * WaveFFT1k<true>::NT];                   // the tile's spectrum, slot-major
+  float2 lbuf[kFtc1kWaves][WaveFFT1k<true>::LDS_ELEMS];  // each wave's exchange buffer
   float2 w2s[64];                                  // W_64^(d * kc), the F2 twiddles
 };
 
 // One work item (tile, chunk of template rows) of a 1024-point job list.
 // `fft` holds this thread's twiddles; `first` is true for the workgroup's
 // first item (wave 0 has just filled w2s: the first barrier publishes it).
-__device__ __forceinline__ void ftc1k_item(Ftc1kLds& L, const WaveFFT1k& fft, const FtcJob* __restrict__ jobs,
-                                           uint32_t n_jobs, uint32_t item) {
-  using F = WaveFFT1k;
+__device__ __forceinline__ void ftc1k_item(Ftc1kLds& L, const WaveFFT1k<true>& fft,
+                                           const FtcJob* __restrict__ jobs, uint32_t n_jobs, uint32_t item) {
+  using F = WaveFFT1k<true>;
   constexpr int NT = F::NT;
   constexpr int S = F::S;
   const int lane = int(threadIdx.x) & 63;
@@ -200,13 +201,64 @@ __device__ __forceinline__ void ftc1k_item(Ftc1kLds& L, const WaveFFT1k& fft, co
   }
 }
 
-__global__ __launch_bounds__(64 * kFtc1kWaves) __attribute__((amdgpu_waves_per_eu(3, 3))) void ftconv1k_kernel(
-    const FtcJob* __restrict__ jobs, uint32_t n_jobs, uint32_t n_items, const float2* __restrict__ tw1,
-    const float2* __restrict__ tw2) {
-  __shared__ Ftc1kLds L;
-  WaveFFT1k fft;
-  fft.load_twiddles(tw1, tw2, L.w2s, int(threadIdx.x) & 63, (threadIdx.x >> 6) == 0);
-  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) ftc1k_item(L, fft, jobs, n_jobs, item);
+// 1024-point tiles, one wave per workgroup.  The tile's spectrum and the
+// twiddles stay in registers; the F2 <-> F3 exchange is a pair of permlane
+// swaps and only the F1 <-> F2 exchange goes through LDS (8.3 KB per
+// workgroup).  160 VGPRs: three waves per SIMD.
+__global__ __launch_bounds__(64) void ftconv1w_kernel(const FtcJob* __restrict__ jobs, uint32_t n_jobs,
+                                                      uint32_t n_items, const float2* __restrict__ tw1,
+                                                      const float2* __restrict__ tw2) {
+  using F = WaveFFT1kLean;
+  constexpr int NT = F::NT;
+  constexpr int S = F::S;
+  __shared__ float2 lb[F::LDS_ELEMS];
+  const int lane = threadIdx.x;
+  F fft;
+  fft.load_twiddles(tw1, tw2, lane);
+
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const FtcJob* job = jobs + find_job_ftc(jobs, n_jobs, item);
+    const uint32_t n_bins = job->n_bins;
+    const uint32_t overlap = job->overlap;
+    const uint32_t payload = job->payload;
+    const uint32_t local = item - job->item_begin;
+    const uint32_t chunk = local / job->n_tiles;
+    const uint32_t tile = local - chunk * job->n_tiles;
+    const float2* __restrict__ x = static_cast<const float2*>(job->spectrum);
+    const uint32_t r_begin = job->row_begin + chunk * job->rows_per_item;
+    uint32_t r_end = r_begin + job->rows_per_item;
+    const uint32_t r_stop = job->row_begin + job->n_rows;
+    if (r_end > r_stop) r_end = r_stop;
+    const float2* __restrict__ hbase = static_cast<const float2*>(job->hperm);
+    const uint32_t out0 = tile * payload;
+
+    float2 xf[16];
+    const int base = int(tile * payload) - int(overlap);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      const int idx = base + NT * a + lane;
+      xf[a] = make_float2(0.f, 0.f);
+      if (idx >= 0 && idx < int(n_bins)) xf[a] = gld2(x, size_t(idx));
+    }
+    fft.forward(xf, lb, lane);
+    for (uint32_t r = r_begin; r < r_end; ++r) {
+      // The template spectrum is not prefetched: at three waves per SIMD the
+      // other waves' transforms cover the load.
+      float2 v[16];
+#pragma unroll
+      for (int s = 0; s < 16; ++s) v[s] = cmul(xf[s], gld2(hbase + size_t(r) * S + lane, size_t(s * NT)));
+      fft.inverse(v, lb, lane);
+      float* __restrict__ out = job->fop + size_t(r) * n_bins + (ptrdiff_t(out0) + lane - ptrdiff_t(overlap));
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        const uint32_t s = uint32_t(NT * a + lane);
+        if (s >= overlap) {
+          const uint32_t f = out0 + s - overlap;
+          if (f < n_bins) gst_stream(out, size_t(NT * a), v[a].x * v[a].x + v[a].y * v[a].y);
+        }
+      }
+    }
+  }
 }
 
 template <int S>
@@ -234,14 +286,14 @@ __global__ __launch_bounds__(S / 16) void tiles_kernel(const float2* __restrict_
   for (int s = 0; s < 16; ++s) o[F::slot_bin(tid, s)] = v[s];
 }
 
-int ftconv_threads(int tile) { return tile == 1024 ? 64 * kFtc1kWaves : tile / 16; }
+int ftconv_threads(int tile) { return tile == 1024 ? 64 : tile / 16; }
 
 hipError_t launch_ftconv(int tile, const FtcJob* d_jobs, uint32_t n_jobs, uint32_t n_items, const float2* tw1,
                          const float2* tw2, uint32_t grid, hipStream_t stream) {
   if (n_items == 0) return hipSuccess;
   if (grid == 0 || grid > n_items) grid = n_items;
   switch (tile) {
-    case 1024: hipLaunchKernelGGL(ftconv1k_kernel, dim3(grid), dim3(64 * kFtc1kWaves), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
+    case 1024: hipLaunchKernelGGL(ftconv1w_kernel, dim3(grid), dim3(64), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
     case 2048: hipLaunchKernelGGL(ftconv_kernel<2048>, dim3(grid), dim3(128), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
     case 4096: hipLaunchKernelGGL(ftconv_kernel<4096>, dim3(grid), dim3(256), 0, stream, d_jobs, n_jobs, n_items, tw1, tw2); break;
     default: return hipErrorInvalidValue;
@@ -601,7 +653,7 @@ __global__ __launch_bounds__(64 * kFtc1kWaves) __attribute__((amdgpu_waves_per_e
   const uint32_t a0 = uint32_t(uint64_t(v) * n_items / total);
   const uint32_t a1 = uint32_t(uint64_t(v + 1) * n_items / total);
   if (a1 > a0) {
-    WaveFFT1k fft;
+    WaveFFT1k<true> fft;
     fft.load_twiddles(tw1, tw2, lds.ftc.w2s, int(threadIdx.x) & 63, (threadIdx.x >> 6) == 0);
     ftc1k_item(lds.ftc, fft, ftc_jobs, n_ftc_jobs, a0);
   } else {

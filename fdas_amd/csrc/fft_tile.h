@@ -394,24 +394,41 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The F2 twiddles W_64^(d * kc) (d = lane / 16) are read from a 64-entry LDS
+// table shared by the workgroup (15 fewer complex registers per thread).
+template <bool W2_LDS = true>
 struct WaveFFT1k {
+  static_assert(W2_LDS, "the F2 twiddles of this form live in LDS");
   static constexpr int S = 1024;
   static constexpr int NT = 64;
   static constexpr int RS1 = NT + FDAS_RS1_PAD;  // row stride of the [ka][b] exchange
   static constexpr int LDS_ELEMS = 16 * RS1;     // one wave's exchange buffer (float2)
 
-  float2 w1[16];                     // W_1024^(lane * ka), in registers
-  const float2* __restrict__ w2;     // LDS copy of W_64^(d * kc) at [kc * 4 + d], d = lane / 16
+  float2 w1[16];                     // W_1024^(lane * ka)
+  float2 w2r[W2_LDS ? 1 : 16];       // W_64^(d * kc), registers
+  const float2* __restrict__ w2;     // or its LDS copy at [kc * 4 + d]
 
   // w2_lds: 64 float2 of LDS the calling wave fills (lanes 0..63 copy one entry
-  // each); it is read after the caller's next barrier.
+  // each) when W2_LDS; it is read after the caller's next barrier.
   __device__ __forceinline__ void load_twiddles(const float2* __restrict__ tw1, const float2* __restrict__ tw2,
                                                 float2* w2_lds, int lane, bool fill) {
     w1[0] = make_float2(1.f, 0.f);
 #pragma unroll
     for (int k = 1; k < 16; ++k) w1[k] = tw1[k * NT + lane];
-    if (fill) w2_lds[lane] = tw2[lane];
-    w2 = w2_lds;
+    if constexpr (W2_LDS) {
+      if (fill) w2_lds[lane] = tw2[lane];
+      w2 = w2_lds;
+    } else {
+      w2r[0] = make_float2(1.f, 0.f);
+#pragma unroll
+      for (int k = 1; k < 16; ++k) w2r[k] = tw2[k * 4 + (lane >> 4)];
+      w2 = nullptr;
+    }
+  }
+
+  __device__ __forceinline__ float2 tw2(int k, int d) const {
+    if constexpr (W2_LDS) return w2[k * 4 + d];
+    else return w2r[k];
   }
 
   // x[a] = tile[64 a + lane] -> spectrum slots (TileFFT<1024>::slot_bin order).
@@ -428,7 +445,7 @@ struct WaveFFT1k {
     for (int c = 0; c < 16; ++c) v[c] = lb[ka * RS1 + 4 * c + d];
     dft16<-1>(v);
 #pragma unroll
-    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], w2[k * 4 + d]);
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], tw2(k, d));
     swap_lane_hi_slot_lo(v);
     dft4<-1, 0, 1>(v);
     dft4<-1, 4, 1>(v);
@@ -445,8 +462,93 @@ struct WaveFFT1k {
     swap_lane_hi_slot_lo(v);
     const int ka = lane & 15, d = lane >> 4;
 #pragma unroll
-    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w2[k * 4 + d]);
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], tw2(k, d));
     dft16<+1>(v);
+    wave_lds_order();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lb[ka * RS1 + 4 * c + d] = v[c];
+    wave_lds_order();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lb[k * RS1 + lane];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], w1[k]);
+    dft16<+1>(v);
+  }
+};
+
+// The one-wave transform with fewer live registers: the F2 twiddles are
+// applied on the other side of the lane exchange, where register 4i + dd of
+// lane (ka, m) needs W_64^(dd (4i + m)) = W_16^(dd i) W_64^(dd m): a
+// compile-time rotation times one of three per-lane values u_dd = W_64^(dd m),
+// m = lane / 16 (6 registers instead of 30; registers with dd = 0 need none).
+struct WaveFFT1kLean {
+  static constexpr int S = 1024;
+  static constexpr int NT = 64;
+  static constexpr int RS1 = NT + FDAS_RS1_PAD;
+  static constexpr int LDS_ELEMS = 16 * RS1;
+
+  float2 w1[16];  // W_1024^(lane * ka)
+  float2 u[4];    // u[dd] = W_64^(dd * (lane / 16)), dd = 1..3
+
+  __device__ __forceinline__ void load_twiddles(const float2* __restrict__ tw1, const float2* __restrict__ tw2,
+                                                int lane) {
+    w1[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) w1[k] = tw1[k * NT + lane];
+    u[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int dd = 1; dd < 4; ++dd) u[dd] = tw2[dd * 4 + (lane >> 4)];  // tw2[k * 4 + d] = W_64^(d k)
+  }
+
+  // v[4i + dd] *= (W_16^(dd i) u_dd) for DIR = -1, its conjugate for DIR = +1.
+  template <int DIR, int I, int DD>
+  __device__ __forceinline__ void tw_one(float2 (&v)[16]) const {
+    const float2 t = DIR < 0 ? cmul(v[4 * I + DD], u[DD]) : cmulc(v[4 * I + DD], u[DD]);
+    v[4 * I + DD] = rot<DIR, 16, DD * I>(t);
+  }
+  template <int DIR, int I>
+  __device__ __forceinline__ void tw_row(float2 (&v)[16]) const {
+    tw_one<DIR, I, 1>(v);
+    tw_one<DIR, I, 2>(v);
+    tw_one<DIR, I, 3>(v);
+  }
+  template <int DIR>
+  __device__ __forceinline__ void twiddle_f2(float2 (&v)[16]) const {
+    tw_row<DIR, 0>(v);
+    tw_row<DIR, 1>(v);
+    tw_row<DIR, 2>(v);
+    tw_row<DIR, 3>(v);
+  }
+
+  __device__ __forceinline__ void forward(float2 (&v)[16], float2* __restrict__ lb, int lane) const {
+    dft16<-1>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], w1[k]);
+    wave_lds_order();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lb[k * RS1 + lane] = v[k];
+    wave_lds_order();
+    const int ka = lane & 15, d = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = lb[ka * RS1 + 4 * c + d];
+    dft16<-1>(v);
+    swap_lane_hi_slot_lo(v);
+    twiddle_f2<-1>(v);
+    dft4<-1, 0, 1>(v);
+    dft4<-1, 4, 1>(v);
+    dft4<-1, 8, 1>(v);
+    dft4<-1, 12, 1>(v);
+  }
+
+  __device__ __forceinline__ void inverse(float2 (&v)[16], float2* __restrict__ lb, int lane) const {
+    dft4<+1, 0, 1>(v);
+    dft4<+1, 4, 1>(v);
+    dft4<+1, 8, 1>(v);
+    dft4<+1, 12, 1>(v);
+    twiddle_f2<+1>(v);
+    swap_lane_hi_slot_lo(v);
+    dft16<+1>(v);
+    const int ka = lane & 15, d = lane >> 4;
     wave_lds_order();
 #pragma unroll
     for (int c = 0; c < 16; ++c) lb[ka * RS1 + 4 * c + d] = v[c];

@@ -272,9 +272,10 @@ typedef struct fdas_engine_config {
   uint64_t hbm_budget_bytes;   /* device memory for per-trial state (default 200 GB) */
   uint32_t ftc_workgroups_per_cu; /* FT-convolution workgroups resident per CU (a persistent grid), leaving
                                      room for the harmonic summing that runs beside it; 0: one per item */
-  uint32_t no_fusion;          /* 0 (default): plans run group g's FT convolution and group g-1's harmonic
-                                  summing in one launch, as one fused kernel where the shapes allow (1024-point
-                                  tiles, 8 harmonics, or 16 with N % 8 == 0); 1: separate kernels per group */
+  uint32_t fused_kernel;       /* plans run group g's FT convolution right after group g-1's harmonic summing
+                                  on one stream; 1: as one fused kernel whose workgroups interleave both kinds
+                                  of work, where the shapes allow (1024-point tiles, 8 harmonics, or 16 with
+                                  N % 8 == 0); 0 (default): as consecutive kernels */
 } fdas_engine_config;
 
 /* Request flags. */
@@ -321,7 +322,7 @@ typedef struct fdas_plan_info {
   float last_ms;               /* device time of the last execution (start to end events) */
   float ftc_ms, hsum_ms;       /* summed FT-convolution / harmonic-summing kernel spans of the last eager
                                   execution, outside fused launches (0 under a graph) */
-  uint32_t lagged;             /* 1: the plan runs the lagged schedule (see no_fusion) */
+  uint32_t lagged;             /* 1: the plan runs the lagged schedule (see fused_kernel) */
   uint32_t n_fused;            /* fused launches in the last eager execution */
   float fused_ms;              /* their summed spans (each: one group's convolution + the previous group's
                                   summing) */

@@ -267,48 +267,55 @@ def test_edge_shapes_against_oracle(banks, n, T, H, part):
         assert_same(s, r)
 
 
-def test_fused_schedule_equals_separate_kernels(banks):
-    """Plans run group g's FT convolution together with group g-1's harmonic summing (one fused
-    kernel for 1024-point tiles and a single summing shape per group; consecutive kernels
-    otherwise).  The lists equal those of the separate-kernel schedule bit for bit, eager and
-    graph-replayed, and meet the oracle's bars."""
+def test_fused_kernel_schedule(banks):
+    """Plans run group g's FT convolution together with group g-1's harmonic summing, as
+    consecutive kernels (default) or as one fused kernel whose workgroups interleave both
+    kinds of work (1024-point tiles, one summing shape per group).  The fused kernel's
+    four-wave transform rounds its products in another order than the one-wave kernel, so
+    both schedules are held to the oracle's bars; each is bit-identical between eager runs
+    and graph replays, and the default schedule's lists equal the single-trial path's."""
     cap = 128
     cases = [  # (n_bins, templates, harmonics): groups of 2 -> fused pairs, a mixed-H group, ragged N
         (1 << 14, 85, 8), (12007, 85, 8), (1 << 14, 169, 16), (9000, 85, 16),
         (1 << 13, 85, 8), (1 << 14, 169, 8), (6007, 169, 16), (1 << 13, 85, 16), (5000, 85, 8),
     ]
-    xs, trials_spec = [], []
+    spec = []
     for i, (n, T, H) in enumerate(cases):
         x = spectrum(n, 700 + i, f0_frac=0.15 + 0.07 * i, z=-40.0 + 9 * i)
-        xs.append(x)
-        trials_spec.append((x, T, synth.noise_thresholds(x, H)))
+        spec.append((x, T, synth.noise_thresholds(x, H)))
     out = {}
-    for fusion in (True, False):
-        with Engine(tile=1024, cands_per_harmonic=cap, group_trials=2, groups_in_flight=2, fusion=fusion,
+    for fused in (True, False):
+        with Engine(tile=1024, cands_per_harmonic=cap, group_trials=2, groups_in_flight=2, fused_kernel=fused,
                     log=quiet) as eng:
             ids = {T: eng.add_bank(banks[T]) for T in banks}
-            trials = [Trial(x, ids[T], thr, 2, tag=i) for i, (x, T, thr) in enumerate(trials_spec)]
+            trials = [Trial(x, ids[T], thr, 2, tag=i) for i, (x, T, thr) in enumerate(spec)]
             plan = eng.plan(trials)
             res = plan.run()
             info = plan.info()
-            assert info["lagged"] == (1 if fusion else 0)
-            if fusion:
-                assert info["n_fused"] >= 2 and info["fused_ms"] > 0
-                graph = eng.plan(trials, graph=True)
-                for _ in range(2):
-                    for a, b in zip(graph.run(), res):
-                        assert_same(a, b)
-                graph.close()
+            assert info["lagged"] == 1
+            assert (info["n_fused"] >= 2) if fused else (info["n_fused"] == 0)
+            assert (info["fused_ms"] > 0) == fused
             plan.close()
-            out[fusion] = res
-    for a, b in zip(out[True], out[False]):
-        assert a.status == 0 and a.tile == 1024
-        assert_same(a, b)
-    for i in (0, 2, 6):
-        x, T, thr = trials_spec[i]
+            graph = eng.plan(trials, graph=True)
+            for _ in range(2):
+                for a, b in zip(graph.run(), res):
+                    assert_same(a, b)
+            graph.close()
+            out[fused] = res
+    for i, (x, T, thr) in enumerate(spec):
         _, _, sel = O.run(x, banks[T], thr, len(thr), cap, "all")
-        r = out[True][i]
-        assert_lists_match(planes_of(r, T), r.n_detected, sel, thr, cap, T // 2)
+        for fused in (True, False):
+            r = out[fused][i]
+            assert r.status == 0 and r.tile == 1024
+            assert_lists_match(planes_of(r, T), r.n_detected, sel, thr, cap, T // 2)
+    # the default schedule equals the single-trial path bit for bit
+    for i in (0, 4):  # 85 templates, 8 harmonics: the reference's 32-bit lists hold them
+        x, T, thr = spec[i]
+        planes, det = single_trial(x, banks[T], thr, 2, 1024, cap)
+        r = out[False][i]
+        assert np.array_equal(r.n_detected, det)
+        for (gt, gb, gp), (et, eb, ep) in zip(planes, planes_of(r, T)):
+            assert np.array_equal(gt, et) and np.array_equal(gb, eb) and np.array_equal(gp, ep)
 
 
 def test_served_requests_can_be_canceled_while_queued(banks):
