@@ -201,6 +201,7 @@ SIGNATURES = {
     "fdas_engine_poll": (_I, [_P, C.POINTER(C.POINTER(Result)), _u32, _U, C.c_int32]),
     "fdas_engine_get_stats": (_I, [_P, C.POINTER(EngineStats)]),
     "fdas_engine_cancel": (_I, [_P, C.c_uint64, _U]),
+    "fdas_hsum_noalloc": (_I, [_P, _u32, _u32, _I, _u32, _F, _u32, _u32, _P, _P, _P, _P, LOG_FN, _P]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "libfdas_amd.so"
 
-SOURCES = ["fdas_kernels.hip", "fdas_frontend.hip", "fdas_api.cpp", "fdas_plan.cpp",
+SOURCES = ["fdas_kernels.hip", "fdas_frontend.hip", "fdas_noalloc.hip", "fdas_api.cpp", "fdas_plan.cpp",
            "fdas_engine.cpp", "fdas_templates.cpp", "fdas_io.cpp"]
 ARCH = os.environ.get("FDAS_OFFLOAD_ARCH", "gfx950")
 

@@ -216,6 +216,33 @@ fdas_status fdas_reference_drifts(uint32_t n_templates, double max_accel, double
                                   double speed_of_light, double* out);
 
 /* ---------------------------------------------------------------------------
+ * Allocation-free harmonic summing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * fdas_hsum_noalloc runs the kernels of fdas_perform_harmonic_summing over a
+ * FOP in the caller's device memory (T x N floats, row = t + T/2) and writes
+ * the caller's device arrays: d_key / d_power [H][cands_per_harmonic]
+ * ((row << 22) | bin and HP_k, sorted by key per plane; entries past a plane's
+ * count are left as they were) and d_counts [2H] (entries kept per plane, then
+ * detections per plane before the cap).  Asynchronous on `stream` (a
+ * hipStream_t, NULL for the default stream); the caller synchronises.  It
+ * calls no allocating routine: job descriptors, spill lists, counters,
+ * histograms and compaction lists live in one device array of
+ * FDAS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  Sizes
+ * that need more (8 bytes per spill entry per plane, plus the cap lists) are
+ * rejected with FDAS_EINVAL and a message giving the limit; spill_per_harmonic
+ * 0 means max(262144, T * N / 512), as in fdas_config.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race; calls in one stream are ordered.  Errors go to `log`
+ * ("[ERROR] ...", may be NULL). */
+#define FDAS_NOALLOC_SCRATCH_BYTES (32u << 20)
+fdas_status fdas_hsum_noalloc(const float* d_fop, uint32_t n_bins, uint32_t n_templates, fdas_part part,
+                              uint32_t n_harmonics, const float* thresholds, uint32_t cands_per_harmonic,
+                              uint32_t spill_per_harmonic, uint32_t* d_key, float* d_power, uint32_t* d_counts,
+                              void* stream, fdas_log_fn log, void* user);
+
+/* ---------------------------------------------------------------------------
  * Front end: time series -> spectrum (upstream of the reference's device path)
  * ------------------------------------------------------------------------- */
 

@@ -159,3 +159,22 @@ def test_engine_cancel_rejects_null_engine():
     lib = N.load()
     assert lib.fdas_engine_cancel(None, 1, None) == 1
     assert lib.fdas_status_string(8) == b"request canceled"
+
+
+def test_noalloc_summing_rejects_oversize_before_device():
+    """fdas_hsum_noalloc checks every size against its fixed scratch before touching a device."""
+    lib = N.load()
+    msgs = []
+    cb = N.LOG_FN(lambda lvl, m, u: msgs.append(m.decode()))
+    thr = np.ones(16, np.float32)
+    dummy = C.c_void_p(0x1000)
+    # 16 planes x 2^20 spill entries x 8 bytes = 128 MiB > the 32 MiB scratch
+    st = lib.fdas_hsum_noalloc(dummy, 1 << 20, 85, 2, 16, thr.ctypes.data_as(N._F), 4096, 1 << 20, dummy, dummy,
+                               dummy, None, cb, None)
+    assert st == 1 and msgs and "scratch" in msgs[-1] and "maximum" in msgs[-1]
+    st = lib.fdas_hsum_noalloc(None, 100, 85, 2, 8, thr.ctypes.data_as(N._F), 64, 0, dummy, dummy, dummy, None, cb,
+                               None)
+    assert st == 1
+    st = lib.fdas_hsum_noalloc(dummy, 100, 84, 2, 8, thr.ctypes.data_as(N._F), 64, 0, dummy, dummy, dummy, None, cb,
+                               None)
+    assert st == 1 and "odd" in msgs[-1]
