@@ -244,7 +244,7 @@ def make_engine(args, **kw):
     from fdas_amd.engine import Engine
     return Engine(tile=args.tile, rows_per_item=args.rows_per_item, hsum_rows=args.hsum_rows,
                   cands_per_harmonic=CAP, group_trials=args.group_trials,
-                  ftc_workgroups_per_cu=args.ftc_wg_per_cu, fused_kernel=args.fused_kernel,
+                  ftc_workgroups_per_cu=args.ftc_wg_per_cu, schedule=args.schedule,
                   log=lambda lvl, msg: (log(msg) if (lvl >= 1 or args.verbose) else None), **kw)
 
 
@@ -536,8 +536,8 @@ def main():
     ap.add_argument("--ftc-wg-per-cu", type=int, default=0,
                     help="FT-convolution workgroups resident per CU (0: one per work item)")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--fused-kernel", action="store_true",
-                    help="plans run each group's FT convolution and the previous group's summing as one fused kernel")
+    ap.add_argument("--schedule", default="lagged", choices=["lagged", "fused", "streams"],
+                    help="plan schedule of each group's FT convolution against the previous group's summing")
     ap.add_argument("--plain-trials", type=int, default=20)
     ap.add_argument("--serve-rates", default="50,100,200,400")
     ap.add_argument("--capacity-requests", type=int, default=400)

@@ -58,7 +58,7 @@ class EngineConfig(C.Structure):
         ("device", C.c_int32),
         ("hbm_budget_bytes", C.c_uint64),
         ("ftc_workgroups_per_cu", C.c_uint32),
-        ("fused_kernel", C.c_uint32),
+        ("plan_schedule", C.c_uint32),
     ]
 
 

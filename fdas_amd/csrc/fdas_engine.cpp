@@ -330,7 +330,7 @@ struct fdas_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = false, launched = false, eager_timed = false;
-  bool fused_kernel = false;  // lagged launches as one fused kernel where the shapes allow
+  uint32_t schedule = FDAS_SCHEDULE_LAGGED;
   uint32_t n_launches = 0;
 };
 
@@ -1241,7 +1241,8 @@ fdas_status fdas_engine_create(const fdas_engine_config* cfg_in, fdas_log_fn log
     return fail(e.get(), FDAS_EINVAL, "tile %u is not 1024, 2048 or 4096", c.tile);
   if (c.rows_per_item > 256) return fail(e.get(), FDAS_EINVAL, "rows_per_item %u above 256", c.rows_per_item);
   if (c.hsum_rows > 256) return fail(e.get(), FDAS_EINVAL, "hsum_rows %u above 256", c.hsum_rows);
-  if (c.fused_kernel > 1) return fail(e.get(), FDAS_EINVAL, "fused_kernel %u is not 0 or 1", c.fused_kernel);
+  if (c.plan_schedule > FDAS_SCHEDULE_STREAMS)
+    return fail(e.get(), FDAS_EINVAL, "plan_schedule %u is not LAGGED, FUSED or STREAMS", c.plan_schedule);
   if (c.ftc_workgroups_per_cu > 64)
     return fail(e.get(), FDAS_EINVAL, "ftc_workgroups_per_cu %u above 64", c.ftc_workgroups_per_cu);
   if (c.cands_per_harmonic < 1 || c.cands_per_harmonic > kSelectMaxCap)
@@ -1421,7 +1422,7 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
   auto p = std::make_unique<fdas_plan>();
   p->eng = e;
   p->use_graph = (flags & FDAS_PLAN_GRAPH) != 0;
-  p->fused_kernel = e->cfg.fused_kernel != 0;
+  p->schedule = e->cfg.plan_schedule;
   p->trials.resize(n);
   for (uint32_t i = 0; i < n; ++i) {
     fdas_status st = make_trial(e, reqs[i], false, &p->trials[i]);
@@ -1570,7 +1571,8 @@ static fdas_status plan_enqueue_lagged(fdas_plan* p, bool capturing) {
     }
     hipEvent_t t0 = cur ? cur->ev_ftc0 : prev->ev_hsum0, t1 = cur ? cur->ev_ftc_t1 : prev->ev_hsum1;
     if (!capturing) EHIP(e, hipEventRecord(t0, A));
-    const bool fuse = p->fused_kernel && cur && prev && cur->ftc.size() == 1 && prev->hsum.size() == 1 &&
+    const bool fuse = p->schedule == FDAS_SCHEDULE_FUSED && cur && prev && cur->ftc.size() == 1 &&
+                      prev->hsum.size() == 1 &&
                       cur->ftc[0].tile == 1024 &&
                       fused_supported(1024, prev->hsum[0].H, prev->hsum[0].vec);
     if (cur) cur->launch_fused = fuse;
@@ -1609,7 +1611,25 @@ static fdas_status plan_enqueue_lagged(fdas_plan* p, bool capturing) {
   return FDAS_OK;
 }
 
-static fdas_status plan_enqueue(fdas_plan* p, bool capturing) { return plan_enqueue_lagged(p, capturing); }
+// Two streams: group g's convolution (stream A) runs beside group g-1's
+// summing, selection and read-back (stream B).
+static fdas_status plan_enqueue_streams(fdas_plan* p, bool capturing) {
+  fdas_engine* e = p->eng;
+  const uint32_t R = uint32_t(p->regions.size());
+  for (uint32_t g = 0; g < p->groups.size(); ++g) {
+    std::vector<Trial*> ptrs;
+    for (uint32_t i : p->groups[g]) ptrs.push_back(&p->trials[i]);
+    const fdas_status st =
+        enqueue_group(e, p->bufs[g], ptrs, g >= R ? p->bufs[g - R].ev_done : nullptr, nullptr, !capturing);
+    if (st != FDAS_OK) return st;
+  }
+  EHIP(e, hipStreamWaitEvent(e->s_ftc, p->bufs.back().ev_done, 0));  // join the summing stream
+  return FDAS_OK;
+}
+
+static fdas_status plan_enqueue(fdas_plan* p, bool capturing) {
+  return p->schedule == FDAS_SCHEDULE_STREAMS ? plan_enqueue_streams(p, capturing) : plan_enqueue_lagged(p, capturing);
+}
 
 fdas_status fdas_plan_launch(fdas_plan* p) {
   if (!p) return FDAS_EINVAL;
@@ -1684,7 +1704,7 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p->ev_start, p->ev_end) == hipSuccess) out->last_ms = ms;
     (void)hipGetLastError();
-    out->lagged = 1u;
+    out->lagged = p->schedule != FDAS_SCHEDULE_STREAMS;
     for (size_t gi = 0; gi < p->bufs.size(); ++gi) {
       if (!p->eager_timed) break;
       const GroupBuf& g = p->bufs[gi];
@@ -1698,7 +1718,8 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
         }
       }
       (void)hipGetLastError();
-      const bool summed_here = gi + 1 == p->bufs.size();  // the last group is summed in a launch of its own
+      // lagged: only the last group's summing has a launch (and timing events) of its own
+      const bool summed_here = !out->lagged || gi + 1 == p->bufs.size();
       if (summed_here && hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
       (void)hipGetLastError();
     }

@@ -20,6 +20,7 @@ import numpy as np
 from . import _native as N
 
 KEY_BIN_BITS = 22
+SCHEDULES = {"lagged": 0, "fused": 1, "streams": 2}  # fdas_engine_config.plan_schedule
 
 
 class EngineError(RuntimeError):
@@ -198,7 +199,7 @@ class Plan:
 class Engine:
     def __init__(self, *, tile=0, rows_per_item=0, hsum_rows=0, cands_per_harmonic=4096, spill_per_harmonic=0,
                  group_trials=8, groups_in_flight=3, device=0, hbm_budget_bytes=200 * 10**9,
-                 ftc_workgroups_per_cu=0, fused_kernel=False, log=None):
+                 ftc_workgroups_per_cu=0, schedule="lagged", log=None):
         self._lib = N.load()
         self._log_target = log
         self._log_cb = N.LOG_FN(self._on_log)
@@ -209,7 +210,7 @@ class Engine:
         cfg.group_trials, cfg.groups_in_flight, cfg.device = group_trials, groups_in_flight, device
         cfg.hbm_budget_bytes = hbm_budget_bytes
         cfg.ftc_workgroups_per_cu = ftc_workgroups_per_cu
-        cfg.fused_kernel = 1 if fused_kernel else 0
+        cfg.plan_schedule = SCHEDULES[schedule]
         self._h = C.c_void_p()
         self.last_error = ""
         st = self._lib.fdas_engine_create(C.byref(cfg), self._log_cb, None, C.byref(self._h))

@@ -299,11 +299,19 @@ typedef struct fdas_engine_config {
   uint64_t hbm_budget_bytes;   /* device memory for per-trial state (default 200 GB) */
   uint32_t ftc_workgroups_per_cu; /* FT-convolution workgroups resident per CU (a persistent grid), leaving
                                      room for the harmonic summing that runs beside it; 0: one per item */
-  uint32_t fused_kernel;       /* plans run group g's FT convolution right after group g-1's harmonic summing
-                                  on one stream; 1: as one fused kernel whose workgroups interleave both kinds
-                                  of work, where the shapes allow (1024-point tiles, 8 harmonics, or 16 with
-                                  N % 8 == 0); 0 (default): as consecutive kernels */
+  uint32_t plan_schedule;      /* how plans order group g's FT convolution against group g-1's harmonic
+                                  summing: FDAS_SCHEDULE_LAGGED (default), _FUSED or _STREAMS */
 } fdas_engine_config;
+
+/* Plan schedules (fdas_engine_config.plan_schedule).
+ *  LAGGED:  one stream; launch g runs group g's FT convolution, then group g-1's summing, back to back.
+ *  FUSED:   as LAGGED, with both in one fused kernel whose workgroups interleave the two kinds of work,
+ *           where the shapes allow (1024-point tiles, 8 harmonics, or 16 with N % 8 == 0).
+ *  STREAMS: convolution on one stream, summing on another; group g's convolution runs beside group
+ *           g-1's summing, sharing the CUs. */
+#define FDAS_SCHEDULE_LAGGED 0u
+#define FDAS_SCHEDULE_FUSED 1u
+#define FDAS_SCHEDULE_STREAMS 2u
 
 /* Request flags. */
 #define FDAS_REQ_DEVICE_SPECTRUM 0x1u /* `spectrum` is device memory (else host memory) */
@@ -349,7 +357,7 @@ typedef struct fdas_plan_info {
   float last_ms;               /* device time of the last execution (start to end events) */
   float ftc_ms, hsum_ms;       /* summed FT-convolution / harmonic-summing kernel spans of the last eager
                                   execution, outside fused launches (0 under a graph) */
-  uint32_t lagged;             /* 1: the plan runs the lagged schedule (see fused_kernel) */
+  uint32_t lagged;             /* 1: the plan runs a lagged schedule (LAGGED or FUSED) */
   uint32_t n_fused;            /* fused launches in the last eager execution */
   float fused_ms;              /* their summed spans (each: one group's convolution + the previous group's
                                   summing) */

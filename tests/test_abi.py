@@ -95,7 +95,7 @@ def test_drift_bank_rejects_too_long_template():
 @pytest.mark.parametrize("field,value", [("tile", 3000), ("cands_per_harmonic", 0), ("cands_per_harmonic", 8193),
                                          ("groups_in_flight", 1), ("groups_in_flight", 9), ("group_trials", 257),
                                          ("spill_per_harmonic", 100), ("rows_per_item", 1000),
-                                         ("fused_kernel", 2)])
+                                         ("plan_schedule", 3)])
 def test_invalid_engine_config_rejected_before_device(field, value):
     lib = N.load()
     cfg = N.EngineConfig()

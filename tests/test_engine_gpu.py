@@ -267,13 +267,14 @@ def test_edge_shapes_against_oracle(banks, n, T, H, part):
         assert_same(s, r)
 
 
-def test_fused_kernel_schedule(banks):
-    """Plans run group g's FT convolution together with group g-1's harmonic summing, as
-    consecutive kernels (default) or as one fused kernel whose workgroups interleave both
-    kinds of work (1024-point tiles, one summing shape per group).  The fused kernel's
-    four-wave transform rounds its products in another order than the one-wave kernel, so
-    both schedules are held to the oracle's bars; each is bit-identical between eager runs
-    and graph replays, and the default schedule's lists equal the single-trial path's."""
+def test_plan_schedules(banks):
+    """Plans order group g's FT convolution against group g-1's harmonic summing as consecutive
+    kernels on one stream (lagged, the default), as one fused kernel whose workgroups
+    interleave both kinds of work (1024-point tiles, one summing shape per group), or on two
+    streams side by side.  The fused kernel's four-wave transform rounds its products in
+    another order than the one-wave kernel, so every schedule is held to the oracle's bars;
+    each is bit-identical between eager runs and graph replays, and the lagged and two-stream
+    schedules' lists equal the single-trial path's."""
     cap = 128
     cases = [  # (n_bins, templates, harmonics): groups of 2 -> fused pairs, a mixed-H group, ragged N
         (1 << 14, 85, 8), (12007, 85, 8), (1 << 14, 169, 16), (9000, 85, 16),
@@ -284,15 +285,16 @@ def test_fused_kernel_schedule(banks):
         x = spectrum(n, 700 + i, f0_frac=0.15 + 0.07 * i, z=-40.0 + 9 * i)
         spec.append((x, T, synth.noise_thresholds(x, H)))
     out = {}
-    for fused in (True, False):
-        with Engine(tile=1024, cands_per_harmonic=cap, group_trials=2, groups_in_flight=2, fused_kernel=fused,
+    for schedule in ("fused", "lagged", "streams"):
+        fused = schedule == "fused"
+        with Engine(tile=1024, cands_per_harmonic=cap, group_trials=2, groups_in_flight=2, schedule=schedule,
                     log=quiet) as eng:
             ids = {T: eng.add_bank(banks[T]) for T in banks}
             trials = [Trial(x, ids[T], thr, 2, tag=i) for i, (x, T, thr) in enumerate(spec)]
             plan = eng.plan(trials)
             res = plan.run()
             info = plan.info()
-            assert info["lagged"] == 1
+            assert info["lagged"] == (0 if schedule == "streams" else 1)
             assert (info["n_fused"] >= 2) if fused else (info["n_fused"] == 0)
             assert (info["fused_ms"] > 0) == fused
             plan.close()
@@ -301,18 +303,20 @@ def test_fused_kernel_schedule(banks):
                 for a, b in zip(graph.run(), res):
                     assert_same(a, b)
             graph.close()
-            out[fused] = res
+            out[schedule] = res
+    for a, b in zip(out["lagged"], out["streams"]):
+        assert_same(a, b)
     for i, (x, T, thr) in enumerate(spec):
         _, _, sel = O.run(x, banks[T], thr, len(thr), cap, "all")
-        for fused in (True, False):
-            r = out[fused][i]
+        for schedule in ("fused", "lagged"):
+            r = out[schedule][i]
             assert r.status == 0 and r.tile == 1024
             assert_lists_match(planes_of(r, T), r.n_detected, sel, thr, cap, T // 2)
     # the default schedule equals the single-trial path bit for bit
     for i in (0, 4):  # 85 templates, 8 harmonics: the reference's 32-bit lists hold them
         x, T, thr = spec[i]
         planes, det = single_trial(x, banks[T], thr, 2, 1024, cap)
-        r = out[False][i]
+        r = out["lagged"][i]
         assert np.array_equal(r.n_detected, det)
         for (gt, gb, gp), (et, eb, ep) in zip(planes, planes_of(r, T)):
             assert np.array_equal(gt, et) and np.array_equal(gb, eb) and np.array_equal(gp, ep)
