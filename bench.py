@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000, calibrated
 # on a copy kernel) for the batched workload at N=2^21, T=85, tile 1024, 8 trials
 # per launch; profiles/batched_n2p21_t85_h8_tile1024_traffic.txt.
-MEASURED_TRAFFIC = {("ftconv_kernel<1024>", 8): 6808.1e6, ("hsum_kernel<8,true>", 8): 9350.6e6}
+MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6854.1e6, ("hsum_kernel<8,true>", 8): 9356.0e6}
 FP32_PEAK_TFLOPS = 157.3
 N_METRIC = 1 << 21
 T_METRIC = 85
@@ -75,8 +75,14 @@ def hsum_bytes(n_bins, n_templates, n_cands, trials=1):
     return trials * (4 * n_templates * n_bins) + 8 * n_cands
 
 
+def path_bytes(n_bins, n_templates):
+    """Algorithmic bytes of one trial through the whole path: spectrum in, FOP written and read back
+    (the template spectra are resident and candidates are a few hundred KB)."""
+    return 8 * n_bins + 8 * n_templates * n_bins
+
+
 def ftc_kernel_name(tile):
-    return "ftconv1k_kernel" if tile == 1024 else f"ftconv_kernel<{tile}>"
+    return "ftconv1w_kernel" if tile == 1024 else f"ftconv_kernel<{tile}>"
 
 
 def ftc_flops(shape, tile):
@@ -293,11 +299,12 @@ def batched_form(args, eng, bank85, workload):
 
         def collect(p):
             info = p.info()
-            if info["lagged"] and info["n_fused"]:
+            n_unfused = info["n_groups"] - info["n_fused"]
+            if info["n_fused"]:
                 spans["fused"].append(info["fused_ms"] / info["n_fused"])
-            else:
-                spans["ftc"].append(info["ftc_ms"] / info["n_groups"])
-                spans["hsum"].append(info["hsum_ms"] / info["n_groups"])
+            if n_unfused:
+                spans["ftc"].append(info["ftc_ms"] / n_unfused)
+                spans["hsum"].append(info["hsum_ms"] / n_unfused)
 
         plans = [eng.plan(trials, graph=graph) for _ in range(2)]
         out["groups_per_batch"] = plans[0].info()["n_groups"]
@@ -627,6 +634,8 @@ def main():
                 "ftconv_fft_tflops": round(trials_per_launch * ftc_flops(shape, tile)
                                            / (kernels[ftc_name]["ms"] * 1e-3) / 1e12, 2),
                 "ftconv_flops_per_trial": int(ftc_flops(shape, tile)),
+                "path_achieved_gbs": round(value * path_bytes(N_METRIC, T_METRIC) / 1e9, 1),
+                "path_frac": round(value * path_bytes(N_METRIC, T_METRIC) / 1e9 / HBM_PEAK_GBS, 4),
                 "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS}
 
     cpu = None

@@ -257,7 +257,8 @@ struct GroupBuf {
   hipEvent_t ev_ftc0 = nullptr, ev_ftc1 = nullptr, ev_hsum0 = nullptr, ev_hsum1 = nullptr, ev_done = nullptr;
   hipEvent_t ev_ftc_t1 = nullptr;
   hipEvent_t ev_hsum_done = nullptr;  // harmonic summing finished (the next group's convolution waits on it)
-  bool launch_fused = false;           // lagged schedule: this group's launch also summed the group before
+  bool launch_fused = false;           // this group's convolution ran in a fused kernel (with the previous summing)
+  bool hsum_fused = false;             // this group's summing ran in the next group's fused kernel
   struct FtcLaunch {
     uint32_t tile, first, n_jobs, n_items;
   };
@@ -1547,9 +1548,7 @@ fdas_status fdas_plan_set_request(fdas_plan* p, uint32_t i, const void* spectrum
 // (g = 0..G) runs group g's FT convolution together with group g-1's harmonic
 // summing, as one fused kernel when the pair's shapes allow it (1024-point
 // tiles, one summing shape), else as consecutive kernels.  Group g-1's
-// selection and read-back follow on stream B beside launch g+1.  Timing events
-// (eager runs only) bracket each launch: ev_ftc0 / ev_ftc_t1 of group g for
-// launches 0..G-1, ev_hsum0 / ev_hsum1 of the last group for the final one.
+// selection and read-back follow on stream B beside launch g+1.
 static fdas_status plan_enqueue_lagged(fdas_plan* p, bool capturing) {
   fdas_engine* e = p->eng;
   hipStream_t A = e->s_ftc, B = e->s_hsum;
@@ -1569,31 +1568,37 @@ static fdas_status plan_enqueue_lagged(fdas_plan* p, bool capturing) {
           EHIP(e, launch_zero_u32(reinterpret_cast<uint32_t*>(t->small), uint32_t(t->lay.small_bytes / 4), A));
       }
     }
-    hipEvent_t t0 = cur ? cur->ev_ftc0 : prev->ev_hsum0, t1 = cur ? cur->ev_ftc_t1 : prev->ev_hsum1;
-    if (!capturing) EHIP(e, hipEventRecord(t0, A));
     const bool fuse = p->schedule == FDAS_SCHEDULE_FUSED && cur && prev && cur->ftc.size() == 1 &&
-                      prev->hsum.size() == 1 &&
-                      cur->ftc[0].tile == 1024 &&
+                      prev->hsum.size() == 1 && cur->ftc[0].tile == 1024 &&
                       fused_supported(1024, prev->hsum[0].H, prev->hsum[0].vec);
     if (cur) cur->launch_fused = fuse;
+    if (prev) prev->hsum_fused = fuse;
+    // Timing events bracket each kernel: the convolution (or the fused kernel)
+    // with ev_ftc0 / ev_ftc_t1 of its group, the summing with ev_hsum0 / ev_hsum1.
+    if (cur && !capturing) EHIP(e, hipEventRecord(cur->ev_ftc0, A));
     if (fuse) {
       const auto& F = cur->ftc[0];
       const auto& S = prev->hsum[0];
       const auto& tw = e->twiddles.at(1024);
       EHIP(e, launch_fused(cur->d_ftc + F.first, F.n_jobs, F.n_items, tw.first, tw.second, prev->d_hsum + S.first,
                            S.n_jobs, S.n_blocks, S.H, S.vec, A));
+      if (!capturing) EHIP(e, hipEventRecord(cur->ev_ftc_t1, A));
     } else {
-      if (cur)
+      if (cur) {
         for (const auto& L : cur->ftc) {
           const auto& tw = e->twiddles.at(L.tile);
           EHIP(e, launch_ftconv(int(L.tile), cur->d_ftc + L.first, L.n_jobs, L.n_items, tw.first, tw.second,
                                 ftc_grid(e, L.n_items), A));
         }
-      if (prev)
+        if (!capturing) EHIP(e, hipEventRecord(cur->ev_ftc_t1, A));
+      }
+      if (prev) {
+        if (!capturing) EHIP(e, hipEventRecord(prev->ev_hsum0, A));
         for (const auto& L : prev->hsum)
           EHIP(e, launch_hsum(prev->d_hsum + L.first, L.n_jobs, L.n_blocks, L.H, L.vec, A));
+        if (!capturing) EHIP(e, hipEventRecord(prev->ev_hsum1, A));
+      }
     }
-    if (!capturing) EHIP(e, hipEventRecord(t1, A));
     if (prev) {
       EHIP(e, hipEventRecord(prev->ev_hsum_done, A));
       EHIP(e, hipStreamWaitEvent(B, prev->ev_hsum_done, 0));
@@ -1705,9 +1710,8 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
     if (hipEventElapsedTime(&ms, p->ev_start, p->ev_end) == hipSuccess) out->last_ms = ms;
     (void)hipGetLastError();
     out->lagged = p->schedule != FDAS_SCHEDULE_STREAMS;
-    for (size_t gi = 0; gi < p->bufs.size(); ++gi) {
+    for (const GroupBuf& g : p->bufs) {
       if (!p->eager_timed) break;
-      const GroupBuf& g = p->bufs[gi];
       float a = 0.f, b = 0.f;
       if (hipEventElapsedTime(&a, g.ev_ftc0, g.ev_ftc_t1) == hipSuccess) {
         if (g.launch_fused) {
@@ -1718,9 +1722,7 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
         }
       }
       (void)hipGetLastError();
-      // lagged: only the last group's summing has a launch (and timing events) of its own
-      const bool summed_here = !out->lagged || gi + 1 == p->bufs.size();
-      if (summed_here && hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
+      if (!g.hsum_fused && hipEventElapsedTime(&b, g.ev_hsum0, g.ev_hsum1) == hipSuccess) out->hsum_ms += b;
       (void)hipGetLastError();
     }
   }

@@ -356,7 +356,7 @@ typedef struct fdas_plan_info {
   uint64_t region_bytes;       /* device bytes of the plan's per-trial state */
   float last_ms;               /* device time of the last execution (start to end events) */
   float ftc_ms, hsum_ms;       /* summed FT-convolution / harmonic-summing kernel spans of the last eager
-                                  execution, outside fused launches (0 under a graph) */
+                                  execution, outside fused kernels (0 under a graph) */
   uint32_t lagged;             /* 1: the plan runs a lagged schedule (LAGGED or FUSED) */
   uint32_t n_fused;            /* fused launches in the last eager execution */
   float fused_ms;              /* their summed spans (each: one group's convolution + the previous group's
