@@ -12,6 +12,7 @@
 //                 (detect.cl.mako:106-115) with a deterministic rule.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 #include "fdas_kernels.h"
@@ -457,13 +458,30 @@ __device__ __forceinline__ void hsum_add_k(int k, float (&hp)[kHsumBinsPerThread
   }
 }
 
-// One test per plane for the common case: the largest of the 8 sums against
-// the threshold; only a wave with a hit looks at the bins one by one.  The
-// thresholds are wave-uniform; `live` is false for a thread whose bins all lie
-// past the end.
+// Key of the per-plane group test: a value v > thr implies
+// float_as_int(v) >= group_key(thr) for every v and thr (a non-negative thr
+// orders like its bit pattern against any v above it; a negative one admits
+// everything), so the test below only ever lets through a superset of the
+// bins the exact float compare accepts.
+__device__ __forceinline__ int group_key(float thr) { return thr < 0.f ? INT_MIN : __float_as_int(thr); }
+
+// Largest bit pattern of the 8 sums as signed integers: three max3 and one max,
+// with no NaN canonicalisation of the operands (which fmaxf would need).
+__device__ __forceinline__ int group_max(const float (&hp)[kHsumBinsPerThread]) {
+  const int a = max(max(__float_as_int(hp[0]), __float_as_int(hp[1])), __float_as_int(hp[2]));
+  const int b = max(max(__float_as_int(hp[3]), __float_as_int(hp[4])), __float_as_int(hp[5]));
+  return max(max(a, b), max(__float_as_int(hp[6]), __float_as_int(hp[7])));
+}
+
+// One test per plane for the common case: the group key of the 8 sums against
+// the plane's; only a wave with a candidate looks at the bins one by one with
+// the exact float compare.  The thresholds are wave-uniform; `live_lanes`
+// holds the lanes with at least one bin in range (per-bin range checks are in
+// `live_bits`, bit b for bin b).
 template <int H>
 __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
-                                         uint32_t f0, const uint32_t (&rk)[H], const float (&thr)[H], bool live,
+                                         uint32_t f0, const uint32_t (&rk)[H], const float (&thr)[H],
+                                         const int (&thr_key)[H], uint64_t live_lanes, uint32_t live_bits,
                                          uint32_t (&wn)[H], uint32_t wave) {
   float hp[kHsumBinsPerThread];
 #pragma unroll
@@ -471,14 +489,13 @@ __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, c
 #pragma unroll
   for (int k = 1; k <= H; ++k) {
     if (k > 1) hsum_add_k<H>(k, hp, L, rk);
-    const float mx = fmaxf(fmaxf(fmaxf(hp[0], hp[1]), fmaxf(hp[2], hp[3])),
-                           fmaxf(fmaxf(hp[4], hp[5]), fmaxf(hp[6], hp[7])));
-    if (__ballot(live && mx > thr[k - 1]) == 0) continue;
+    if ((__ballot(group_max(hp) >= thr_key[k - 1]) & live_lanes) == 0) continue;
     // The wave's hits in lane order, each lane's in bin order: one prefix sum
     // over the per-lane counts places them all.
     uint32_t bits = 0;
 #pragma unroll
-    for (int b = 0; b < kHsumBinsPerThread; ++b) bits |= (live && hp[b] > thr[k - 1]) ? (1u << b) : 0u;
+    for (int b = 0; b < kHsumBinsPerThread; ++b) bits |= (hp[b] > thr[k - 1]) ? (1u << b) : 0u;
+    bits &= live_bits;
     uint32_t total;
     uint32_t pos = wave_prefix_small(uint32_t(__builtin_popcount(bits)), total);
     if (wn[k - 1] + total > uint32_t(kHsumWaveStage)) {
@@ -538,23 +555,27 @@ __device__ __forceinline__ void hsum_block(HsumStage<H>& st, const HsumJob* __re
   const uint32_t r_begin = job.row_begin + chunk * job.rows_per_block;
   const uint32_t r_last = min(r_begin + job.rows_per_block, job.row_begin + job.n_rows) - 1;
   float thr[H];
+  int thr_key[H];
   uint32_t wn[H], rk[H];
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     thr[k] = job.thr[k];
+    thr_key[k] = group_key(thr[k]);
     wn[k] = 0;
     rk[k] = fl % uint32_t(k + 1);
   }
 
   const bool live = f0 < n_bins;
+  const uint64_t live_lanes = __ballot(live);
+  const uint32_t live_bits = live ? 0xffu : 0u;  // bins past the end read as -inf (or lie in the clamped copy)
   RowLoads<H> a, b;
   hsum_load_row<H, VEC>(a, fop, n_bins, c, r_begin, fl);
   for (uint32_t r = r_begin;; r += 2) {
     hsum_load_row<H, VEC>(b, fop, n_bins, c, min(r + 1, r_last), fl);
-    hsum_row<H>(st, job, a, r, fl, rk, thr, live, wn, wave);
+    hsum_row<H>(st, job, a, r, fl, rk, thr, thr_key, live_lanes, live_bits, wn, wave);
     if (r + 1 > r_last) break;
     hsum_load_row<H, VEC>(a, fop, n_bins, c, min(r + 2, r_last), fl);
-    hsum_row<H>(st, job, b, r + 1, fl, rk, thr, live, wn, wave);
+    hsum_row<H>(st, job, b, r + 1, fl, rk, thr, thr_key, live_lanes, live_bits, wn, wave);
     if (r + 2 > r_last) break;
   }
   // Flush what is left: one global atomic per plane for the whole workgroup.
