@@ -344,16 +344,16 @@ __device__ __forceinline__ uint32_t wave_prefix_small(uint32_t v, uint32_t& tota
 // flushed with one atomic per plane per workgroup.
 template <int H>
 struct HsumStage {
-  uint32_t key[kHsumThreads / 64][H][kHsumWaveStage];
-  float pwr[kHsumThreads / 64][H][kHsumWaveStage];
+  static constexpr int kCap = kHsumWaveStage;
+  uint32_t key[kHsumThreads / 64][H][kCap];
+  float pwr[kHsumThreads / 64][H][kCap];
   uint32_t n[kHsumThreads / 64][H];
   uint32_t base[kHsumThreads / 64][H];
 };
 
 // Writes the wave's n staged entries of plane p to the global spill list.
-template <int H>
-__device__ __forceinline__ void wave_flush(HsumStage<H>& st, const HsumJob& job, uint32_t wave, uint32_t p,
-                                           uint32_t n) {
+template <class Stage>
+__device__ __forceinline__ void wave_flush(Stage& st, const HsumJob& job, uint32_t wave, uint32_t p, uint32_t n) {
   const uint32_t lane = lane_id();
   uint32_t g = 0;
   if (lane == 0) g = gatomic_add(job.counts + p, n);
@@ -398,23 +398,30 @@ struct RowLoads {
 //    (f0 / 2 is a multiple of 4); the k = 4 pair is one 8-byte load.
 //  otherwise: one load per value, each index clamped to the last bin, and the
 //    own bins past the end read as -inf, so no sum of theirs passes a threshold.
-template <int H, bool VEC>
-__device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __restrict__ fop, uint32_t n_bins, int c,
-                                              uint32_t r, uint32_t f0) {
+template <bool VEC>
+__device__ __forceinline__ void hsum_load_own(float (&m)[kHsumBinsPerThread], const float* __restrict__ fop,
+                                              uint32_t n_bins, uint32_t r, uint32_t f0) {
   const float* __restrict__ row = fop + size_t(r) * n_bins;
   const uint32_t last = n_bins - 1;
   if constexpr (VEC) {
     const fdas_vf4 q0 = gld4(row, f0);
     const fdas_vf4 q1 = gld4(row, f0 + 4);
-    L.m[0] = q0.x; L.m[1] = q0.y; L.m[2] = q0.z; L.m[3] = q0.w;
-    L.m[4] = q1.x; L.m[5] = q1.y; L.m[6] = q1.z; L.m[7] = q1.w;
+    m[0] = q0.x; m[1] = q0.y; m[2] = q0.z; m[3] = q0.w;
+    m[4] = q1.x; m[5] = q1.y; m[6] = q1.z; m[7] = q1.w;
   } else {
 #pragma unroll
     for (int b = 0; b < kHsumBinsPerThread; ++b) {
       const float v = gld(row, min(f0 + uint32_t(b), last));
-      L.m[b] = (f0 + uint32_t(b) < n_bins) ? v : -INFINITY;
+      m[b] = (f0 + uint32_t(b) < n_bins) ? v : -INFINITY;
     }
   }
+}
+
+// The sub-harmonic spans of row r (harmonics 2..H) for the 8 bins from f0.
+template <int H, bool VEC>
+__device__ __forceinline__ void hsum_load_subs(RowLoads<H>& L, const float* __restrict__ fop, uint32_t n_bins, int c,
+                                               uint32_t r, uint32_t f0) {
+  const uint32_t last = n_bins - 1;
   const int t = int(r) - c;
 #pragma unroll
   for (int k = 2; k <= H; ++k) {
@@ -431,6 +438,13 @@ __device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __res
       for (int j = 0; j < hsum_span(k); ++j) L.s[k - 1][j] = gld(srow, VEC ? s0 + uint32_t(j) : min(s0 + uint32_t(j), last));
     }
   }
+}
+
+template <int H, bool VEC>
+__device__ __forceinline__ void hsum_load_row(RowLoads<H>& L, const float* __restrict__ fop, uint32_t n_bins, int c,
+                                              uint32_t r, uint32_t f0) {
+  hsum_load_own<VEC>(L.m, fop, n_bins, r, f0);
+  hsum_load_subs<H, VEC>(L, fop, n_bins, c, r, f0);
 }
 
 // Bin b of the thread takes sub-harmonic value (f0 mod k + b) / k of its span:
@@ -478,8 +492,8 @@ __device__ __forceinline__ int group_max(const float (&hp)[kHsumBinsPerThread]) 
 // the exact float compare.  The thresholds are wave-uniform; `live_lanes`
 // holds the lanes with at least one bin in range (per-bin range checks are in
 // `live_bits`, bit b for bin b).
-template <int H>
-__device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
+template <int H, class Stage>
+__device__ __forceinline__ void hsum_row(Stage& st, const HsumJob& job, const RowLoads<H>& L, uint32_t r,
                                          uint32_t f0, const uint32_t (&rk)[H], const float (&thr)[H],
                                          const int (&thr_key)[H], uint64_t live_lanes, uint32_t live_bits,
                                          uint32_t (&wn)[H], uint32_t wave) {
@@ -498,11 +512,11 @@ __device__ __forceinline__ void hsum_row(HsumStage<H>& st, const HsumJob& job, c
     bits &= live_bits;
     uint32_t total;
     uint32_t pos = wave_prefix_small(uint32_t(__builtin_popcount(bits)), total);
-    if (wn[k - 1] + total > uint32_t(kHsumWaveStage)) {
+    if (wn[k - 1] + total > uint32_t(Stage::kCap)) {
       wave_flush(st, job, wave, uint32_t(k - 1), wn[k - 1]);
       wn[k - 1] = 0;
     }
-    if (total > uint32_t(kHsumWaveStage)) {
+    if (total > uint32_t(Stage::kCap)) {
       // more than a stage holds: straight to the spill list (entries past its end are counted, not kept)
       uint32_t g = 0;
       if (lane_id() == 0) g = gatomic_add(job.counts + (k - 1), total);
