@@ -64,6 +64,8 @@ struct DevSet {
   hipEvent_t ev[EV_COUNT] = {};
   hipEvent_t staged_ftc = nullptr;   // h_ftc may be rewritten once this completes
   hipEvent_t staged_hsum = nullptr;  // h_hsum / h_sel may be rewritten once this completes
+  hipEvent_t hsum_done = nullptr;    // the set's last harmonic-summing kernel has finished
+  bool hsum_issued = false;          // hsum_done has been recorded at least once
   bool ftc_pending = false;
   bool hsum_pending = false;
   bool ev_recorded[EV_COUNT] = {};
@@ -227,6 +229,12 @@ fdas_status enqueue_ftc(fdas_ctx* ctx, DevSet& s, fdas_part part) {
   HIPCK(ctx, hipMemcpyAsync(s.d_ftc, s.h_ftc, sizeof(FtcJob) * jobs.size(), hipMemcpyHostToDevice, s.stream));
   HIPCK(ctx, hipEventRecord(s.staged_ftc, s.stream));
   s.ftc_pending = true;
+  // The two sets' kernels take turns: this convolution starts once the other
+  // set's last summing kernel has finished (both kinds of kernel are VALU-bound,
+  // and side by side they only slow each other down), while that set's
+  // selection and read-back, and this set's input copy, still overlap.
+  const DevSet& other = &s == &ctx->set[0] ? ctx->set[1] : ctx->set[0];
+  if (other.hsum_issued) HIPCK(ctx, hipStreamWaitEvent(s.stream, other.hsum_done, 0));
   if ((st = record(ctx, s, EV_FTC_K)) != FDAS_OK) return st;
   HIPCK(ctx, launch_ftconv(int(ctx->cfg.tile), s.d_ftc, uint32_t(jobs.size()), n_items, ctx->tw1, ctx->tw2,
                            n_items, s.stream));
@@ -281,6 +289,8 @@ fdas_status enqueue_hsum(fdas_ctx* ctx, DevSet& s, fdas_part part, const float* 
   if ((st = record(ctx, s, EV_HSUM_K0)) != FDAS_OK) return st;
   HIPCK(ctx, launch_hsum(s.d_hsum, 1, blocks, H, hsum_vec_ok(ctx->geo.n_bins), s.stream));
   if ((st = record(ctx, s, EV_HSUM_K1)) != FDAS_OK) return st;
+  HIPCK(ctx, hipEventRecord(s.hsum_done, s.stream));
+  s.hsum_issued = true;
   HIPCK(ctx, launch_select(s.d_sel, H, s.stream));
   return FDAS_OK;
 }
@@ -368,6 +378,7 @@ void free_set(DevSet& s) {
     if (e) (void)hipEventDestroy(e);
   if (s.staged_ftc) (void)hipEventDestroy(s.staged_ftc);
   if (s.staged_hsum) (void)hipEventDestroy(s.staged_hsum);
+  if (s.hsum_done) (void)hipEventDestroy(s.hsum_done);
   if (s.stream) (void)hipStreamDestroy(s.stream);
   s = DevSet{};
 }
@@ -511,7 +522,8 @@ extern "C" fdas_status fdas_create(const fdas_config* cfg, fdas_log_fn log, void
     for (auto& ev : s.ev)
       if (hipEventCreate(&ev) != hipSuccess) return bail(fail(ctx, FDAS_EDEVICE, "event creation failed"));
     if (hipEventCreateWithFlags(&s.staged_ftc, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s.staged_hsum, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&s.staged_hsum, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.hsum_done, hipEventDisableTiming) != hipSuccess)
       return bail(fail(ctx, FDAS_EDEVICE, "event creation failed"));
     // a FOP that was never written reads as zeros
     if (hipMemset(s.fop, 0, T * N * sizeof(float)) != hipSuccess ||
