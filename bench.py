@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000, calibrated
 # on a copy kernel) for the batched workload at N=2^21, T=85, tile 1024, 8 trials
 # per launch; profiles/batched_n2p21_t85_h8_tile1024_traffic.txt.
-MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6854.1e6, ("hsum_kernel<8,true>", 8): 9356.0e6}
+MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6853.7e6, ("hsum_kernel<8,true>", 8): 9299.1e6}
 FP32_PEAK_TFLOPS = 157.3
 N_METRIC = 1 << 21
 T_METRIC = 85
