@@ -554,8 +554,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the FDAS hot path shards DM trials across GPUs with no "
-              "collective, so a multi-GPU figure is N independent copies of this run (see DESIGN.md)",
+        print("bench.py: measures one GPU only; the FDAS hot path would shard DM trials across GPUs with no "
+              "collective (each GPU runs its own trials; candidate lists merge on the host), so a multi-GPU "
+              "figure would be N independent copies of this run",
               file=sys.stderr)
         sys.exit(2)
     forms = set(args.forms.split(","))
