@@ -33,7 +33,7 @@ typedef enum fdas_status {
   FDAS_EINVAL = 1,        /* bad argument or size */
   FDAS_ENOMEM = 2,        /* device or host allocation failed, or HBM budget exceeded */
   FDAS_EDEVICE = 3,       /* a HIP call failed */
-  FDAS_EUNSUPPORTED = 4,  /* valid request outside what this build or format supports */
+  FDAS_EUNSUPPORTED = 4,  /* valid request outside what this library or the format supports */
   FDAS_EBUSY = 5,         /* resource in use (e.g. serving queue full) */
   FDAS_ESTATE = 6,        /* call out of order (e.g. convolution before templates) */
   FDAS_EOVERFLOW = 7,     /* detections exceeded the spill capacity; lists are incomplete */

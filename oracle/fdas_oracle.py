@@ -6,7 +6,7 @@ reference's own CPU path for Fourier-domain acceleration search:
   spectrum (ft_conv.py) -> overlap-save FT convolution -> filter-output plane
   (FOP) -> harmonic summing (hsum.py) -> thresholding -> candidate lists.
 
-It is the checker, never the thing measured or shipped: only ``tests/``,
+It is test infrastructure, never the thing measured or shipped: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 import it.  The GPU product path in ``fdas_amd`` never calls into this file.
 
@@ -427,7 +427,7 @@ def select(cands: list[PlaneCandidates], cap: int) -> list[PlaneCandidates]:
     """Cap each plane at ``cap`` entries: strongest power first, ties by (row, bin).
 
     The reference FPGA keeps a 64-slot ring per plane and silently overwrites
-    (detect.cl.mako:106-115); this build replaces that with a deterministic
+    (detect.cl.mako:106-115); the library replaces that with a deterministic
     rule so that GPU and oracle lists can be compared entry for entry.  The
     kept entries are returned sorted by (row, bin).
     """

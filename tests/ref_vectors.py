@@ -46,7 +46,7 @@ def write_vectors(vec_dir: Path, x: np.ndarray, bank: np.ndarray, n_harmonics: i
 
 
 def driver_tree(root: Path) -> Path:
-    """The directory layout the driver expects: it reads the template file from
+    """The directory layout the reference test driver expects: it reads the template file from
     ../../fdas/tmpl/ relative to its working directory (main_gtest.cpp:503).
     Returns that working directory."""
     tmpl = root / "fdas" / "tmpl"

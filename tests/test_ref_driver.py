@@ -1,10 +1,10 @@
 """The reference's own test driver (fdas/src/host/main_gtest.cpp) against this library.
 
-* CPU: the driver compiles unchanged from /root/reference, with its vendored googletest and
+* CPU: the reference test driver compiles unchanged from /root/reference, with its vendored googletest and
   libnpy, against tests/cpp/refshim/FDAS.h (a drop-in FDAS.h over include/FDAS.hpp) and links
   with libfdas_amd.so (oracle/Makefile, target ref-driver).  Skipped where /root/reference is
   absent.
-* GPU: the driver built that way (oracle/_ref/fdas_gtest, built by __graft_entry__.build())
+* GPU: the reference test driver built that way (oracle/_ref/fdas_gtest, built by __graft_entry__.build())
   runs its serial and pipelined flows (three buffer modes each, main_gtest.cpp:315-437) on
   test vectors written in its own layout at the emulator size (N = 32,768, the reference's
   85-template bank), and its validator (main_gtest.cpp:192-228) accepts every list.
@@ -37,7 +37,7 @@ def test_reference_driver_compiles_against_the_shim(tmp_path):
 
 
 def test_reference_vectors_layout(tmp_path):
-    """The vector writer's files load with the dtypes and shapes the driver asserts."""
+    """The vector writer's files load with the dtypes and shapes the reference test driver asserts."""
     x = synth.emulator_workload().spectrum()[:6000]
     bank = np.load(ROOT / "tests" / "golden" / "fdas_templates_85_350.npy", allow_pickle=False)
     info = write_vectors(tmp_path, x, bank)
