@@ -297,6 +297,8 @@ def test_plan_schedules(banks):
             assert info["lagged"] == (0 if schedule == "streams" else 1)
             assert (info["n_fused"] >= 2) if fused else (info["n_fused"] == 0)
             assert (info["fused_ms"] > 0) == fused
+            if not fused:  # eager runs time each kernel on its own, also when both share one stream
+                assert info["ftc_ms"] > 0 and info["hsum_ms"] > 0
             plan.close()
             graph = eng.plan(trials, graph=True)
             for _ in range(2):
