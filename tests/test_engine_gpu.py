@@ -9,6 +9,10 @@
   keeps device state within the HBM budget.
 """
 
+import random
+import threading
+import time
+
 import numpy as np
 import pytest
 
@@ -349,3 +353,65 @@ def test_served_requests_can_be_canceled_while_queued(banks):
                 for a, c in zip(r.keys, alone.keys):
                     assert np.array_equal(a, c)
         assert eng.stats()["completed"] == eng.stats()["submitted"]
+
+
+def test_served_concurrent_submitters_and_cancels(banks):
+    """Two threads submit mixed shapes (N 2^18..2^20 and ragged, banks 85/169, H 8/16) while a
+    third withdraws recent tags, under an HBM budget that makes admission wait.  Every request
+    completes exactly once, as FDAS_ECANCELED when a cancel withdrew it (the counts agree), else
+    with the lists its spectrum gives alone, bit for bit."""
+    cap = 128
+    shapes = [(1 << 18, 85, 8), (1 << 19, 169, 8), (1 << 20, 85, 16), (1 << 18, 169, 16), (200003, 85, 8)]
+    with Engine(cands_per_harmonic=cap, group_trials=4, groups_in_flight=3, hbm_budget_bytes=6 * 10**9,
+                log=quiet) as eng:
+        ids = {T: eng.add_bank(banks[T]) for T in banks}
+        specs = []
+        for i, (n, T, H) in enumerate(shapes):
+            x = spectrum(n, 5000 + i)
+            eng.prepare(n, ids[T], 2)
+            specs.append((x, T, synth.noise_thresholds(x, H)))
+        alone = [eng.run([Trial(x, ids[T], thr, 2, tag=0)])[0] for (x, T, thr) in specs]
+
+        per_thread = 150
+        sent, lock, stop, withdrawn = {}, threading.Lock(), threading.Event(), [0]
+
+        def submitter(seed, base):
+            rnd = random.Random(seed)
+            for j in range(per_thread):
+                s = rnd.randrange(len(specs))
+                x, T, thr = specs[s]
+                with lock:
+                    sent[base + j] = s
+                eng.submit(Trial(x, ids[T], thr, 2, tag=base + j))
+                if rnd.random() < 0.3:
+                    time.sleep(rnd.random() * 0.002)
+
+        def canceller():
+            rnd = random.Random(7)
+            while not stop.is_set():
+                with lock:
+                    tags = list(sent)[-20:]
+                if tags:
+                    withdrawn[0] += eng.cancel(rnd.choice(tags))
+                time.sleep(0.001)
+
+        threads = [threading.Thread(target=submitter, args=(1, 100000)),
+                   threading.Thread(target=submitter, args=(2, 200000)),
+                   threading.Thread(target=canceller)]
+        for t in threads:
+            t.start()
+        done, t0 = [], time.time()
+        while len(done) < 2 * per_thread and time.time() - t0 < 60:
+            done.extend(eng.poll(max_results=64, timeout_ms=1000))
+        stop.set()
+        for t in threads:
+            t.join()
+        assert len(done) == 2 * per_thread
+        assert len({r.tag for r in done}) == 2 * per_thread
+        canceled = [r for r in done if r.status == 8]
+        assert len(canceled) == withdrawn[0]
+        for r in done:
+            if r.status != 8:
+                assert_same(r, alone[sent[r.tag]])
+        st = eng.stats()
+        assert st["completed"] == st["submitted"] and st["arena_in_use_bytes"] == 0
