@@ -53,6 +53,11 @@ HBM_PEAK_GBS = 8000.0
 # on a copy kernel) for the batched workload at N=2^21, T=85, tile 1024, 8 trials
 # per launch; profiles/batched_n2p21_t85_h8_tile1024_traffic.txt.
 MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6853.7e6, ("hsum_kernel<8,true>", 8): 9299.1e6}
+# VALU wave-instructions per launch (rocprofv3 SQ_INSTS_VALU) at the same shape;
+# profiles/batched_n2p21_t85_h8_tile1024_instructions.txt.  Issue rate: 256 CUs x
+# 4 SIMDs, one wave64 VALU instruction per 4 cycles per SIMD at 2.4 GHz.
+MEASURED_VALU = {("ftconv1w_kernel", 8): 1093471224, ("hsum_kernel<8,true>", 8): 510989269}
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 FP32_PEAK_TFLOPS = 157.3
 N_METRIC = 1 << 21
 T_METRIC = 85
@@ -637,7 +642,15 @@ def main():
                 "ftconv_flops_per_trial": int(ftc_flops(shape, tile)),
                 "path_achieved_gbs": round(value * path_bytes(N_METRIC, T_METRIC) / 1e9, 1),
                 "path_frac": round(value * path_bytes(N_METRIC, T_METRIC) / 1e9 / HBM_PEAK_GBS, 4),
-                "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS}
+                "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS,
+                "valu_issue": ({k: {"instructions": MEASURED_VALU[(k, trials_per_launch)],
+                                    "issue_ms": round(MEASURED_VALU[(k, trials_per_launch)] / VALU_ISSUE_PER_S * 1e3, 4),
+                                    "frac_of_span": round(MEASURED_VALU[(k, trials_per_launch)] / VALU_ISSUE_PER_S
+                                                          / (v["ms"] * 1e-3), 3)}
+                                for k, v in kernels.items() if (k, trials_per_launch) in MEASURED_VALU}
+                               if batched is not None else None),
+                "valu_source": ("profiles/batched_n2p21_t85_h8_tile1024_instructions.txt (rocprofv3 SQ_INSTS_VALU "
+                                "per launch)") if batched is not None else None}
 
     cpu = None
     if not args.no_cpu_baseline:
