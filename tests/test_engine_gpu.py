@@ -415,3 +415,33 @@ def test_served_concurrent_submitters_and_cancels(banks):
                 assert_same(r, alone[sent[r.tag]])
         st = eng.stats()
         assert st["completed"] == st["submitted"] and st["arena_in_use_bytes"] == 0
+
+
+def test_tuning_cache_shared_between_engines(banks, tmp_path):
+    """An engine appends the shape it tunes to its cache file (malformed and invalid lines are
+    skipped); a second engine reading the file takes that shape without timing anything, and gives
+    the same lists for the same request, bit for bit."""
+    path = tmp_path / "tune.txt"
+    path.write_text("not a tuning line\nsomekey tile=999 rows_per_item=1 hsum_rows=1\n")
+    n = 1 << 16
+    x = spectrum(n, 901)
+    thr = synth.noise_thresholds(x, 8)
+    logs_a, logs_b = [], []
+    with Engine(cands_per_harmonic=64, log=lambda level, msg: logs_a.append(msg)) as a:
+        a.tune_cache(path)
+        ba = a.add_bank(banks[85])
+        a.prepare(n, ba, 2)
+        shape_a = a.shape_info(n, ba, 2)
+        ra = a.run([Trial(x, ba, thr, 2)])[0]
+    assert any("autotune" in m for m in logs_a)
+    lines = path.read_text().splitlines()
+    assert len(lines) == 3 and f"tile={shape_a['tile']} rows_per_item={shape_a['rows_per_item']}" in lines[-1]
+    with Engine(cands_per_harmonic=64, log=lambda level, msg: logs_b.append(msg)) as b:
+        b.tune_cache(path)
+        bb = b.add_bank(banks[85])
+        b.prepare(n, bb, 2)
+        assert b.shape_info(n, bb, 2) == shape_a
+        rb = b.run([Trial(x, bb, thr, 2)])[0]
+    assert any("(tuning cache)" in m for m in logs_b) and not any("autotune" in m for m in logs_b)
+    assert len(path.read_text().splitlines()) == 3  # nothing new was timed
+    assert_same(ra, rb)
