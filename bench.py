@@ -52,11 +52,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000, calibrated
 # on a copy kernel) for the batched workload at N=2^21, T=85, tile 1024, 8 trials
 # per launch; profiles/batched_n2p21_t85_h8_tile1024_traffic.txt.
-MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6853.7e6, ("hsum_kernel<8,true>", 8): 9299.1e6}
+MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6869.7e6, ("hsum_kernel<8,true>", 8): 9302.7e6}
 # VALU wave-instructions per launch (rocprofv3 SQ_INSTS_VALU) at the same shape;
 # profiles/batched_n2p21_t85_h8_tile1024_instructions.txt.  Issue rate: 256 CUs x
 # 4 SIMDs, one wave64 VALU instruction per 4 cycles per SIMD at 2.4 GHz.
-MEASURED_VALU = {("ftconv1w_kernel", 8): 1093471224, ("hsum_kernel<8,true>", 8): 510989269}
+MEASURED_VALU = {("ftconv1w_kernel", 8): 1050558056, ("hsum_kernel<8,true>", 8): 510989349}
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 FP32_PEAK_TFLOPS = 157.3
 N_METRIC = 1 << 21

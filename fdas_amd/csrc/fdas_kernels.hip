@@ -205,7 +205,7 @@ __device__ __forceinline__ void ftc1k_item(Ftc1kLds& L, const WaveFFT1k<true>& f
 // 1024-point tiles, one wave per workgroup.  The tile's spectrum and the
 // twiddles stay in registers; the F2 <-> F3 exchange is a pair of permlane
 // swaps and only the F1 <-> F2 exchange goes through LDS (8.3 KB per
-// workgroup).  160 VGPRs: three waves per SIMD.
+// workgroup).  164 VGPRs: three waves per SIMD.
 __global__ __launch_bounds__(64) void ftconv1w_kernel(const FtcJob* __restrict__ jobs, uint32_t n_jobs,
                                                       uint32_t n_items, const float2* __restrict__ tw1,
                                                       const float2* __restrict__ tw2) {
