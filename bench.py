@@ -29,6 +29,8 @@ Also reported in the same JSON line:
   * served_form: open-loop Poisson request streams through the serving loop,
     p50 / p99 latency and throughput beside the plain one-request-per-launch
     form on the same arrivals, and the sustained capacity of both;
+  * power_form (only with --forms ...,power): board power and shader clock
+    while the FT convolution alone, then the summing alone, runs;
   * roofline of the dominant kernel (HIP-event spans on its own stream over
     the timed region) and the CPU baseline (the oracle's C restatement timed
     on this host: all threads and one thread, at N = 2^21 and N = 32,768).
@@ -58,6 +60,9 @@ MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6869.7e6, ("hsum_kernel<8,true>", 8)
 # 4 SIMDs, one wave64 VALU instruction per 4 cycles per SIMD at 2.4 GHz.
 MEASURED_VALU = {("ftconv1w_kernel", 8): 1050558056, ("hsum_kernel<8,true>", 8): 510989349}
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+# Shader clock the kernel runs at (amd-smi median while it runs alone, bench.py --forms single,power);
+# profiles/power_clock_n2p21_t85_tile1024.txt.  The FT convolution holds the board at its power limit.
+MEASURED_CLOCK_GHZ = {"ftconv1w_kernel": 2.096, "hsum_kernel<8,true>": 2.394}
 FP32_PEAK_TFLOPS = 157.3
 N_METRIC = 1 << 21
 T_METRIC = 85
@@ -105,6 +110,20 @@ def device_thresholds(x, n_harmonics, sigma=6.0):
     p = float((x.real.double() ** 2 + x.imag.double() ** 2).mean().item())
     k = np.arange(1, n_harmonics + 1, dtype=np.float64)
     return ((k + sigma * np.sqrt(k)) * p).astype(np.float32)
+
+
+def valu_issue(kernel, trials_per_launch, span_ms):
+    """VALU issue time of one launch at 2.4 GHz and at the clock the kernel was measured running at,
+    each as a fraction of the measured span."""
+    n = MEASURED_VALU[(kernel, trials_per_launch)]
+    at_peak = n / VALU_ISSUE_PER_S
+    out = {"instructions": n, "issue_ms": round(at_peak * 1e3, 4), "frac_of_span": round(at_peak / (span_ms * 1e-3), 3)}
+    ghz = MEASURED_CLOCK_GHZ.get(kernel)
+    if ghz:
+        at_clk = at_peak * 2.4 / ghz
+        out.update({"clock_ghz": ghz, "issue_ms_at_clock": round(at_clk * 1e3, 4),
+                    "frac_of_span_at_clock": round(at_clk / (span_ms * 1e-3), 3)})
+    return out
 
 
 def pct(a, q):
@@ -246,6 +265,87 @@ def single_trial_forms(args, x, bank, thr, forms, tile, device_trials):
             "one trial per launch")
         log(f"plain (host upload): {out['plain_gpu_form_h2d']}")
     f.close()
+    return out
+
+
+def _smi_sample(smi):
+    """(board power W, mean shader clock MHz, min, max) from one `amd-smi metric -p -c` call, or None."""
+    import re
+    import subprocess
+    try:
+        txt = subprocess.run([smi, "metric", "-g", "0", "-p", "-c"], capture_output=True, text=True,
+                             timeout=5).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    pw = re.search(r"SOCKET_POWER:\s*(\d+)", txt)
+    clk = [int(c) for c in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*(\d+)", txt)]
+    if pw is None or not clk:
+        return None
+    return float(pw.group(1)), float(np.mean(clk)), min(clk), max(clk)
+
+
+def power_form(x, bank, tile, seconds):
+    """Board power and shader clock while the FT convolution alone, then the harmonic summing alone,
+    runs back to back on one N = 2^21 trial through the class FDAS path (`amd-smi metric -p -c`
+    sampled about every 0.3 s; medians over each phase after its first 2 s).  The summing phase uses
+    thresholds that no bin passes, so that selection has nothing to sort."""
+    import shutil
+
+    import torch
+    from fdas_amd.host import FDAS
+    smi = shutil.which("amd-smi")
+    if smi is None:
+        return {"error": "amd-smi not found"}
+    f = FDAS(log=lambda lvl, msg: (log(msg) if lvl >= 1 else None))
+    assert f.initialise_accelerator(N_METRIC, tile=tile, n_templates=T_METRIC, max_taps=421,
+                                    n_harmonics=H_METRIC, cands_per_harmonic=CAP), f.last_error
+    assert f.upload_template_taps(bank, FDAS.A), f.last_error
+    assert f.perform_input_tiling(x, FDAS.A), f.last_error
+    none_pass = np.full(H_METRIC, np.finfo(np.float32).max, np.float32)
+    samples, stop = [], threading.Event()
+
+    def sampler():
+        while not stop.is_set():
+            t = time.time()
+            s = _smi_sample(smi)
+            if s is not None:
+                samples.append((t,) + s)
+            stop.wait(0.25)
+
+    th = threading.Thread(target=sampler, daemon=True)
+    th.start()
+    time.sleep(2.0)
+    phases = {"idle": (time.time() - 1.5, time.time(), 0)}
+    steps = (("ftconv", lambda: f.perform_ft_convolution(FDAS.AllAccelerations, FDAS.A)),
+             ("hsum", lambda: f.perform_harmonic_summing(none_pass, FDAS.AllAccelerations, FDAS.A)))
+    for name, fn in steps:
+        assert fn(), f.last_error
+        torch.cuda.synchronize()
+        t0, k = time.time(), 0
+        while time.time() - t0 < seconds:
+            for _ in range(100):
+                assert fn(), f.last_error
+            torch.cuda.synchronize()
+            k += 100
+        phases[name] = (t0, time.time(), k)
+        log(f"power: {name} phase done, {k} launches")
+        time.sleep(1.5)
+    stop.set()
+    th.join()
+    f.close()
+    out = {"sampler": "amd-smi metric -g 0 -p -c", "n_bins": N_METRIC, "tile": tile}
+    for name, (t0, t1, k) in phases.items():
+        lo = t0 if name == "idle" else t0 + 2.0
+        sel = [s for s in samples if lo <= s[0] <= t1]
+        rec = {"samples": len(sel)}
+        if k:
+            rec["launches"] = k
+            rec["ms_per_launch"] = round((t1 - t0) * 1e3 / k, 4)
+        if sel:
+            rec["power_w"] = float(np.median([s[1] for s in sel]))
+            rec["gfx_clk_mhz"] = float(np.median([s[2] for s in sel]))
+            rec["gfx_clk_mhz_range"] = [min(s[3] for s in sel), max(s[4] for s in sel)]
+        out[name] = rec
     return out
 
 
@@ -540,7 +640,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=64, help="trials per batch")
     ap.add_argument("--forms", default="single,plain,batched,uneven,serve",
-                    help="comma list of single, plain, batched, uneven, serve")
+                    help="comma list of single, plain, batched, uneven, serve, power (power: not by default)")
+    ap.add_argument("--power-seconds", type=float, default=8.0, help="length of each phase of the power form")
     ap.add_argument("--tile", type=int, default=0, help="engine tile size (0: autotuned per shape)")
     ap.add_argument("--rows-per-item", type=int, default=0)
     ap.add_argument("--hsum-rows", type=int, default=0)
@@ -591,6 +692,10 @@ def main():
     if "serve" in forms:
         served = served_form(args, eng, banks, {85: bank85, 169: bank169})
     eng.close()
+    power = None
+    if "power" in forms:
+        power = power_form(x, bank85, tile, args.power_seconds)
+        log(f"power: {power}")
     del workload_trials
 
     if batched is not None:
@@ -643,14 +748,12 @@ def main():
                 "path_achieved_gbs": round(value * path_bytes(N_METRIC, T_METRIC) / 1e9, 1),
                 "path_frac": round(value * path_bytes(N_METRIC, T_METRIC) / 1e9 / HBM_PEAK_GBS, 4),
                 "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS,
-                "valu_issue": ({k: {"instructions": MEASURED_VALU[(k, trials_per_launch)],
-                                    "issue_ms": round(MEASURED_VALU[(k, trials_per_launch)] / VALU_ISSUE_PER_S * 1e3, 4),
-                                    "frac_of_span": round(MEASURED_VALU[(k, trials_per_launch)] / VALU_ISSUE_PER_S
-                                                          / (v["ms"] * 1e-3), 3)}
+                "valu_issue": ({k: valu_issue(k, trials_per_launch, v["ms"])
                                 for k, v in kernels.items() if (k, trials_per_launch) in MEASURED_VALU}
                                if batched is not None else None),
                 "valu_source": ("profiles/batched_n2p21_t85_h8_tile1024_instructions.txt (rocprofv3 SQ_INSTS_VALU "
-                                "per launch)") if batched is not None else None}
+                                "per launch); clocks: profiles/power_clock_n2p21_t85_tile1024.txt (amd-smi)")
+                if batched is not None else None}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -686,6 +789,7 @@ def main():
         "plain_gpu_form_h2d": plain_h2d,
         "uneven_form": uneven,
         "served_form": served,
+        "power_form": power,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
