@@ -268,20 +268,26 @@ def single_trial_forms(args, x, bank, thr, forms, tile, device_trials):
     return out
 
 
-def _smi_sample(smi):
-    """(board power W, mean shader clock MHz, min, max) from one `amd-smi metric -p -c` call, or None."""
+def parse_smi_metric(txt):
+    """(board power W, mean shader clock MHz, min, max) from the text of `amd-smi metric -p -c`, or
+    None when either field is missing."""
     import re
+    pw = re.search(r"SOCKET_POWER:\s*(\d+)", txt)
+    clk = [int(c) for c in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*(\d+)", txt)]
+    if pw is None or not clk:
+        return None
+    return float(pw.group(1)), float(np.mean(clk)), min(clk), max(clk)
+
+
+def _smi_sample(smi):
+    """One `amd-smi metric -g 0 -p -c` call, parsed; None if it fails."""
     import subprocess
     try:
         txt = subprocess.run([smi, "metric", "-g", "0", "-p", "-c"], capture_output=True, text=True,
                              timeout=5).stdout
     except (OSError, subprocess.SubprocessError):
         return None
-    pw = re.search(r"SOCKET_POWER:\s*(\d+)", txt)
-    clk = [int(c) for c in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*(\d+)", txt)]
-    if pw is None or not clk:
-        return None
-    return float(pw.group(1)), float(np.mean(clk)), min(clk), max(clk)
+    return parse_smi_metric(txt)
 
 
 def power_form(x, bank, tile, seconds):
