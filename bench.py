@@ -60,9 +60,9 @@ MEASURED_TRAFFIC = {("ftconv1w_kernel", 8): 6869.7e6, ("hsum_kernel<8,true>", 8)
 # 4 SIMDs, one wave64 VALU instruction per 4 cycles per SIMD at 2.4 GHz.
 MEASURED_VALU = {("ftconv1w_kernel", 8): 1050558056, ("hsum_kernel<8,true>", 8): 510989349}
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
-# Shader clock the kernel runs at (amd-smi median while it runs alone, bench.py --forms single,power);
+# Shader clock the kernel runs at (amd-smi median while it runs alone, bench.py --forms batched,power);
 # profiles/power_clock_n2p21_t85_tile1024.txt.  The FT convolution holds the board at its power limit.
-MEASURED_CLOCK_GHZ = {"ftconv1w_kernel": 2.096, "hsum_kernel<8,true>": 2.394}
+MEASURED_CLOCK_GHZ = {"ftconv1w_kernel": 2.112, "hsum_kernel<8,true>": 2.385}
 FP32_PEAK_TFLOPS = 157.3
 N_METRIC = 1 << 21
 T_METRIC = 85
@@ -290,11 +290,12 @@ def _smi_sample(smi):
     return parse_smi_metric(txt)
 
 
-def power_form(x, bank, tile, seconds):
+def power_form(x, bank, tile, seconds, batched_plan=None):
     """Board power and shader clock while the FT convolution alone, then the harmonic summing alone,
-    runs back to back on one N = 2^21 trial through the class FDAS path (`amd-smi metric -p -c`
-    sampled about every 0.3 s; medians over each phase after its first 2 s).  The summing phase uses
-    thresholds that no bin passes, so that selection has nothing to sort."""
+    runs back to back on one N = 2^21 trial through the class FDAS path, and then, given a plan of the
+    batched form, while that plan runs batch after batch (`amd-smi metric -p -c` sampled about every
+    0.3 s; medians over each phase after its first 2 s).  The summing phase uses thresholds that no
+    bin passes, so that selection has nothing to sort."""
     import shutil
 
     import torch
@@ -322,17 +323,24 @@ def power_form(x, bank, tile, seconds):
     th.start()
     time.sleep(2.0)
     phases = {"idle": (time.time() - 1.5, time.time(), 0)}
-    steps = (("ftconv", lambda: f.perform_ft_convolution(FDAS.AllAccelerations, FDAS.A)),
-             ("hsum", lambda: f.perform_harmonic_summing(none_pass, FDAS.AllAccelerations, FDAS.A)))
-    for name, fn in steps:
+    def batch():
+        batched_plan.launch()
+        batched_plan.wait(results=False)
+        return True
+
+    steps = [("ftconv", lambda: f.perform_ft_convolution(FDAS.AllAccelerations, FDAS.A), 100),
+             ("hsum", lambda: f.perform_harmonic_summing(none_pass, FDAS.AllAccelerations, FDAS.A), 100)]
+    if batched_plan is not None:
+        steps.append(("batched", batch, 4))
+    for name, fn, per_sync in steps:
         assert fn(), f.last_error
         torch.cuda.synchronize()
         t0, k = time.time(), 0
         while time.time() - t0 < seconds:
-            for _ in range(100):
+            for _ in range(per_sync):
                 assert fn(), f.last_error
             torch.cuda.synchronize()
-            k += 100
+            k += per_sync
         phases[name] = (t0, time.time(), k)
         log(f"power: {name} phase done, {k} launches")
         time.sleep(1.5)
@@ -347,6 +355,8 @@ def power_form(x, bank, tile, seconds):
         if k:
             rec["launches"] = k
             rec["ms_per_launch"] = round((t1 - t0) * 1e3 / k, 4)
+            if name == "batched":
+                rec["trials_per_s"] = round(k * batched_plan.n / (t1 - t0), 1)
         if sel:
             rec["power_w"] = float(np.median([s[1] for s in sel]))
             rec["gfx_clk_mhz"] = float(np.median([s[2] for s in sel]))
@@ -697,11 +707,17 @@ def main():
         uneven = uneven_form(args, eng, banks)
     if "serve" in forms:
         served = served_form(args, eng, banks, {85: bank85, 169: bank169})
-    eng.close()
     power = None
     if "power" in forms:
-        power = power_form(x, bank85, tile, args.power_seconds)
+        plan = None
+        if workload_trials:
+            from fdas_amd.engine import Trial
+            plan = eng.plan([Trial(xd, banks[85], th, 2, tag=i) for i, (xd, th) in enumerate(workload_trials)])
+        power = power_form(x, bank85, tile, args.power_seconds, plan)
+        if plan is not None:
+            plan.close()
         log(f"power: {power}")
+    eng.close()
     del workload_trials
 
     if batched is not None:
