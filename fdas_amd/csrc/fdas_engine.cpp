@@ -312,7 +312,7 @@ struct fdas_engine {
   };
   std::vector<Ctx> ctxs;
   uint64_t launch_seq = 0;
-  uint32_t live_plans = 0;         // plans holding arena memory (guarded by api_mu)
+  std::vector<fdas_plan*> plans;   // live plans, which hold arena memory (guarded by api_mu)
   hipEvent_t last_hsum = nullptr;  // summing-done event of the last served group
   fdas_engine_stats stats{};
 };
@@ -1089,7 +1089,7 @@ bool admit(fdas_engine* e, fdas_engine::Ctx& c, bool others_in_flight, bool& mem
       // Impossible when the trial alone exceeds the budget, or when nothing
       // else holds memory (no group in flight, no live plan) and it still
       // does not fit.  Otherwise it waits for memory to come back.
-      const bool never = t.lay.total > e->arena->budget() || (picked.empty() && !others_in_flight && e->live_plans == 0);
+      const bool never = t.lay.total > e->arena->budget() || (picked.empty() && !others_in_flight && e->plans.empty());
       if (never) {
         {
           // fdas_engine_cancel may have withdrawn it meanwhile
@@ -1304,6 +1304,8 @@ fdas_status fdas_engine_create(const fdas_engine_config* cfg_in, fdas_log_fn log
   return FDAS_OK;
 }
 
+static void plan_release(fdas_engine* e, fdas_plan* p);
+
 void fdas_engine_destroy(fdas_engine* e) {
   if (!e) return;
   if (e->running) {
@@ -1315,6 +1317,11 @@ void fdas_engine_destroy(fdas_engine* e) {
     e->scheduler.join();
   }
   if (e->s_ftc) (void)hipSetDevice(e->cfg.device);
+  {
+    // Plans the caller did not destroy go with the engine; their handles become invalid.
+    std::lock_guard<std::mutex> lk(e->api_mu);
+    while (!e->plans.empty()) plan_release(e, e->plans.back());
+  }
   if (e->s_ftc) (void)hipStreamSynchronize(e->s_ftc);
   if (e->s_hsum) (void)hipStreamSynchronize(e->s_hsum);
   for (auto& c : e->ctxs) {
@@ -1488,6 +1495,8 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
     for (char* m : p->inputs) e->arena->free(m);
     for (auto& b : p->bufs) free_group(b);
     (void)hipHostFree(p->h_read);
+    if (p->ev_start) (void)hipEventDestroy(p->ev_start);
+    if (p->ev_end) (void)hipEventDestroy(p->ev_end);
     return st;
   };
   for (uint32_t r = 0; r < n_regions; ++r) {
@@ -1550,7 +1559,7 @@ fdas_status fdas_plan_create(fdas_engine* e, const fdas_request* reqs, uint32_t 
     return cleanup(fail(e, FDAS_EDEVICE, "event creation failed"));
   info(e, "plan: %u trials in %u groups, %u groups resident (%.2f GB)", n, n_groups, n_regions,
        double(p->region_bytes) / 1e9);
-  e->live_plans += 1;
+  e->plans.push_back(p.get());
   *out = p.release();
   return FDAS_OK;
 }
@@ -1768,10 +1777,8 @@ fdas_status fdas_plan_get_info(const fdas_plan* p, fdas_plan_info* out) {
   return FDAS_OK;
 }
 
-void fdas_plan_destroy(fdas_plan* p) {
-  if (!p) return;
-  fdas_engine* e = p->eng;
-  std::lock_guard<std::mutex> lk(e->api_mu);
+// Releases a plan's device state and removes it from its engine.  Caller holds api_mu.
+static void plan_release(fdas_engine* e, fdas_plan* p) {
   (void)hipSetDevice(e->cfg.device);
   if (p->launched) (void)hipEventSynchronize(p->ev_end);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -1783,8 +1790,17 @@ void fdas_plan_destroy(fdas_plan* p) {
   (void)hipHostFree(p->h_read);
   if (p->ev_start) (void)hipEventDestroy(p->ev_start);
   if (p->ev_end) (void)hipEventDestroy(p->ev_end);
-  e->live_plans -= 1;
+  e->plans.erase(std::find(e->plans.begin(), e->plans.end(), p));
   delete p;
+}
+
+void fdas_plan_destroy(fdas_plan* p) {
+  if (!p) return;
+  fdas_engine* e = p->eng;
+  {
+    std::lock_guard<std::mutex> lk(e->api_mu);
+    plan_release(e, p);
+  }
   e->q_cv.notify_all();  // a served trial may be waiting for the memory just returned
 }
 

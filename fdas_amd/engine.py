@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import sys
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -138,6 +139,7 @@ class Plan:
         self._h = C.c_void_p()
         engine._check(self._lib.fdas_plan_create(engine._h, reqs, len(trials), N.PLAN_GRAPH if graph else 0,
                                                  C.byref(self._h)))
+        engine._plans.add(self)
         self.n = len(trials)
         self._out = [_Outputs(t.n_harmonics, engine.cap) for t in trials]
         self._res = (N.Result * self.n)()
@@ -212,6 +214,7 @@ class Engine:
         cfg.ftc_workgroups_per_cu = ftc_workgroups_per_cu
         cfg.plan_schedule = SCHEDULES[schedule]
         self._h = C.c_void_p()
+        self._plans = weakref.WeakSet()  # closed before the engine: fdas_engine_destroy frees its plans
         self.last_error = ""
         st = self._lib.fdas_engine_create(C.byref(cfg), self._log_cb, None, C.byref(self._h))
         if st != N.FDAS_OK:
@@ -318,6 +321,8 @@ class Engine:
 
     def close(self):
         if self._h:
+            for p in list(self._plans):
+                p.close()
             self._lib.fdas_engine_destroy(self._h)
             self._h = C.c_void_p()
             self._pending.clear()

@@ -367,7 +367,8 @@ typedef struct fdas_plan_info {
 
 void fdas_engine_default_config(fdas_engine_config* cfg);
 fdas_status fdas_engine_create(const fdas_engine_config* cfg, fdas_log_fn log, void* user, fdas_engine** out);
-/* Completes every submitted trial, then releases all device state. */
+/* Completes every submitted trial, destroys the plans still alive (their handles become invalid:
+ * do not pass them to fdas_plan_destroy afterwards), then releases all device state. */
 void fdas_engine_destroy(fdas_engine* eng);
 const char* fdas_engine_last_error(const fdas_engine* eng);
 
@@ -417,6 +418,7 @@ fdas_status fdas_plan_launch(fdas_plan* plan);
 /* Waits for the launch and fills results[0..n) (NULL: only waits). */
 fdas_status fdas_plan_wait(fdas_plan* plan, fdas_result* results);
 fdas_status fdas_plan_get_info(const fdas_plan* plan, fdas_plan_info* info);
+/* Waits for the plan's launch and releases its memory; only before fdas_engine_destroy of its engine. */
 void fdas_plan_destroy(fdas_plan* plan);
 
 /* Serving: trials are queued, admitted while the HBM budget and the groups in

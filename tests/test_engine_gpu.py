@@ -328,6 +328,32 @@ def test_plan_schedules(banks):
             assert np.array_equal(gt, et) and np.array_equal(gb, eb) and np.array_equal(gp, ep)
 
 
+def test_engine_close_releases_open_plans(banks):
+    """Closing an engine with plans still open (one finished, one in flight, one graph-captured)
+    destroys them with it: their handles become inert and later calls fail cleanly.  A plan that
+    an exception leaves open inside `with Engine(...)` takes this path."""
+    x = spectrum(1 << 16, 31)
+    thr = synth.noise_thresholds(x, 8)
+    eng = Engine(tile=1024, cands_per_harmonic=32, log=quiet)
+    b = eng.add_bank(banks[85])
+    done = eng.plan([Trial(x, b, thr, 2)])
+    ref = done.run()[0]
+    running = eng.plan([Trial(x, b, thr, 2), Trial(x, b, thr, 1)])
+    running.launch()
+    captured = eng.plan([Trial(x, b, thr, 0)], graph=True)
+    assert captured.run()[0].status == 0
+    eng.close()
+    for p in (done, running, captured):
+        assert not p._h
+        p.close()  # a no-op now
+    with pytest.raises(EngineError):
+        done.launch()
+    # a new engine in the same process works and gives the same lists
+    with Engine(tile=1024, cands_per_harmonic=32, log=quiet) as eng2:
+        (r,) = eng2.run([Trial(x, eng2.add_bank(banks[85]), thr, 2)])
+        assert_same(r, ref)
+
+
 def test_served_requests_can_be_canceled_while_queued(banks):
     """Requests withdrawn while still queued complete at once with FDAS_ECANCELED; the others
     complete normally with the lists they give alone."""
